@@ -1,0 +1,766 @@
+// libpm C ABI: stateless reference-function mirrors and the device-resident PM simulation.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pm.h"
+#include "pm_fft.h"
+#include "pm_particles.h"
+
+using namespace pm;
+
+// ------------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------------
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define PM_HIP(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail(PM_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+std::mutex g_mu;
+bool g_init = false;
+int g_device = 0;
+hipStream_t g_stream = nullptr;
+
+int ensure_init() {
+  if (g_init) return PM_OK;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_init) return PM_OK;
+  int ndev = 0;
+  PM_HIP(hipGetDeviceCount(&ndev));
+  if (ndev <= 0) return fail(PM_E_HIP, "no HIP device visible");
+  PM_HIP(hipGetDevice(&g_device));
+  PM_HIP(hipStreamCreate(&g_stream));
+  g_init = true;
+  return PM_OK;
+}
+
+#define PM_INIT()                  \
+  do {                             \
+    int rc_ = ensure_init();       \
+    if (rc_ != PM_OK) return rc_;  \
+  } while (0)
+
+// ------------------------------------------------------------------------------------------
+// per-size FFT tables and complex128 workspaces
+// ------------------------------------------------------------------------------------------
+std::map<std::pair<int, int>, FftTables> g_tables;
+
+int get_tables(int n, int prec, FftTables** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_pair(n, prec);
+  auto it = g_tables.find(key);
+  if (it != g_tables.end()) {
+    *out = &it->second;
+    return PM_OK;
+  }
+  std::vector<double> tw(2 * (size_t)n), s2(n), wd(n);
+  for (int m = 0; m < n; ++m) {
+    const double ang = 2.0 * M_PI * (double)m / (double)n;
+    tw[2 * m] = std::cos(ang);
+    tw[2 * m + 1] = -std::sin(ang);
+    const double sp = std::sin(M_PI * (double)m / (double)n);
+    s2[m] = sp * sp;
+    const int mm = m < n / 2 ? m : m - n;
+    const double x = M_PI * (double)mm / (double)n;
+    const double sinc = mm == 0 ? 1.0 : std::sin(x) / x;
+    wd[m] = 1.0 / (sinc * sinc * sinc * sinc);
+  }
+  FftTables t;
+  t.n = n;
+  t.prec = prec;
+  const size_t rb = (size_t)prec;
+  PM_HIP(hipMalloc(&t.tw, 2 * (size_t)n * rb));
+  PM_HIP(hipMalloc(&t.sin2, (size_t)n * rb));
+  PM_HIP(hipMalloc(&t.wdec, (size_t)n * rb));
+  if (prec == 8) {
+    PM_HIP(hipMemcpy(t.tw, tw.data(), tw.size() * 8, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(t.sin2, s2.data(), s2.size() * 8, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(t.wdec, wd.data(), wd.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> a(tw.begin(), tw.end()), b(s2.begin(), s2.end()), c(wd.begin(), wd.end());
+    PM_HIP(hipMemcpy(t.tw, a.data(), a.size() * 4, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(t.sin2, b.data(), b.size() * 4, hipMemcpyHostToDevice));
+    PM_HIP(hipMemcpy(t.wdec, c.data(), c.size() * 4, hipMemcpyHostToDevice));
+  }
+  auto res = g_tables.emplace(key, t);
+  *out = &res.first->second;
+  return PM_OK;
+}
+
+struct Workspace {
+  double2* H = nullptr;
+  double2* Q = nullptr;
+  float* phi = nullptr;
+};
+std::map<int, Workspace> g_ws;
+
+int get_workspace(int n, bool need_phi, Workspace** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Workspace& w = g_ws[n];
+  const size_t nn = (size_t)n;
+  if (!w.H) {
+    if (hipMalloc(&w.H, nn * nn * (nn / 2) * sizeof(double2)) != hipSuccess)
+      return fail(PM_E_NOMEM, "cannot allocate the complex128 half spectrum for N=%d", n);
+    if (hipMalloc(&w.Q, nn * nn * sizeof(double2)) != hipSuccess)
+      return fail(PM_E_NOMEM, "cannot allocate the Nyquist plane for N=%d", n);
+  }
+  if (need_phi && !w.phi) {
+    if (hipMalloc(&w.phi, nn * nn * nn * sizeof(float)) != hipSuccess)
+      return fail(PM_E_NOMEM, "cannot allocate the potential scratch for N=%d", n);
+  }
+  *out = &w;
+  return PM_OK;
+}
+
+int check_grid(int n) {
+  if (!fft_size_supported(n))
+    return fail(PM_E_INVALID, "N_CELLS=%d is not a supported grid size (2^a 3^b between 8 and 1024)", n);
+  return PM_OK;
+}
+
+double f_of_a(double a, double c0, double c1, double c2) {
+  // cosmology.py:20-27
+  return 1.0 / std::sqrt((c0 + c2 * a + c1 * a * a * a) / a);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// library / memory
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+PM_API int pm_init(int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_init) {
+    if (device >= 0 && device != g_device) return fail(PM_E_STATE, "library already bound to device %d", g_device);
+    return PM_OK;
+  }
+  int ndev = 0;
+  PM_HIP(hipGetDeviceCount(&ndev));
+  if (ndev <= 0) return fail(PM_E_HIP, "no HIP device visible");
+  if (device < 0) PM_HIP(hipGetDevice(&device));
+  if (device >= ndev) return fail(PM_E_INVALID, "device %d out of range (%d visible)", device, ndev);
+  PM_HIP(hipSetDevice(device));
+  g_device = device;
+  PM_HIP(hipStreamCreate(&g_stream));
+  g_init = true;
+  return PM_OK;
+}
+
+PM_API const char* pm_last_error(void) { return g_err.c_str(); }
+PM_API int pm_version(void) { return 100; }
+PM_API int pm_supported_grid(int n_cells) { return fft_size_supported(n_cells) ? 1 : 0; }
+
+PM_API int pm_malloc(void** ptr, size_t bytes) {
+  PM_INIT();
+  if (!ptr) return fail(PM_E_INVALID, "null output pointer");
+  if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) return fail(PM_E_NOMEM, "hipMalloc(%zu) failed", bytes);
+  return PM_OK;
+}
+PM_API int pm_free(void* ptr) {
+  PM_INIT();
+  if (ptr) PM_HIP(hipFree(ptr));
+  return PM_OK;
+}
+PM_API int pm_memcpy_htod(void* dst, const void* src, size_t bytes) {
+  PM_INIT();
+  if (bytes) PM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return PM_OK;
+}
+PM_API int pm_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+  PM_INIT();
+  if (bytes) PM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return PM_OK;
+}
+PM_API int pm_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+  PM_INIT();
+  if (bytes) PM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return PM_OK;
+}
+PM_API int pm_memset(void* dst, int value, size_t bytes) {
+  PM_INIT();
+  if (bytes) PM_HIP(hipMemset(dst, value, bytes));
+  return PM_OK;
+}
+PM_API int pm_synchronize(void) {
+  PM_INIT();
+  PM_HIP(hipDeviceSynchronize());
+  return PM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// reference-function mirrors
+// ------------------------------------------------------------------------------------------
+PM_API int pm_density(const float* pos, int64_t n_p, int n, double mass, float* rho) {
+  PM_INIT();
+  if (n <= 0 || n_p < 0 || !rho || (n_p > 0 && !pos)) return fail(PM_E_INVALID, "pm_density: bad arguments");
+  PM_HIP(ref_density(pos, n_p, n, mass, rho, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_fourier_grid(int n, float* g) {
+  PM_INIT();
+  if (n <= 0 || !g) return fail(PM_E_INVALID, "pm_fourier_grid: bad arguments");
+  PM_HIP(ref_fourier_grid(n, g, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_density_k(const float* rho, int n, void* rho_k) {
+  PM_INIT();
+  if (int rc = check_grid(n)) return rc;
+  if (!rho || !rho_k) return fail(PM_E_INVALID, "pm_density_k: null pointer");
+  FftTables* t;
+  Workspace* w;
+  if (int rc = get_tables(n, 8, &t)) return rc;
+  if (int rc = get_workspace(n, false, &w)) return rc;
+  PM_HIP((fft_r2c_x<double, float>(rho, w->H, w->Q, n, *t, g_stream)));
+  PM_HIP(fft_c2c_y<double>(w->H, w->Q, n, -1, *t, g_stream));
+  PM_HIP(fft_c2c_z<double>(w->H, w->Q, n, -1, *t, g_stream));
+  PM_HIP(spec_expand_full(w->H, w->Q, (double2*)rho_k, n, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_potential_k(const void* rho_k, const float* fgrid, int n, double a, double omega_m0, void* phi_k) {
+  PM_INIT();
+  if (n <= 0 || !rho_k || !fgrid || !phi_k || a == 0.0) return fail(PM_E_INVALID, "pm_potential_k: bad arguments");
+  const double c = -3 * omega_m0 / 8 / a;  // potential.py:15
+  PM_HIP(spec_potential_k((const double2*)rho_k, fgrid, c, (double2*)phi_k, n, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_potential_real(const void* phi_k, int n, float* phi) {
+  PM_INIT();
+  if (int rc = check_grid(n)) return rc;
+  if (!phi_k || !phi) return fail(PM_E_INVALID, "pm_potential_real: null pointer");
+  FftTables* t;
+  Workspace* w;
+  if (int rc = get_tables(n, 8, &t)) return rc;
+  if (int rc = get_workspace(n, false, &w)) return rc;
+  const double scale = 1.0 / ((double)n * n * n);  // pyFFTW normalise_idft=True
+  PM_HIP(spec_hermitian_half((const double2*)phi_k, w->H, w->Q, n, scale, g_stream));
+  PM_HIP(fft_c2c_z<double>(w->H, w->Q, n, +1, *t, g_stream));
+  PM_HIP(fft_c2c_y<double>(w->H, w->Q, n, +1, *t, g_stream));
+  PM_HIP((fft_c2r_x<double, float>(w->H, w->Q, phi, n, *t, g_stream)));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+static int potential_into(const float* rho, const float* fgrid, int n, double a, double omega_m0, float* phi) {
+  FftTables* t;
+  Workspace* w;
+  if (int rc = get_tables(n, 8, &t)) return rc;
+  if (int rc = get_workspace(n, false, &w)) return rc;
+  GreenArgs g{};
+  g.coef = (-3 * omega_m0 / 8 / a) / ((double)n * n * n);
+  g.from_grid = 1;
+  g.fgrid = fgrid;
+  PM_HIP((fft_r2c_x<double, float>(rho, w->H, w->Q, n, *t, g_stream)));
+  PM_HIP(fft_c2c_y<double>(w->H, w->Q, n, -1, *t, g_stream));
+  PM_HIP(fft_z_green<double>(w->H, w->Q, n, g, *t, g_stream));
+  PM_HIP(fft_c2c_y<double>(w->H, w->Q, n, +1, *t, g_stream));
+  PM_HIP((fft_c2r_x<double, float>(w->H, w->Q, phi, n, *t, g_stream)));
+  return PM_OK;
+}
+
+PM_API int pm_potential(const float* rho, const float* fgrid, int n, double a, double omega_m0, float* phi) {
+  PM_INIT();
+  if (int rc = check_grid(n)) return rc;
+  if (!rho || !fgrid || !phi || a == 0.0) return fail(PM_E_INVALID, "pm_potential: bad arguments");
+  if (int rc = potential_into(rho, fgrid, n, a, omega_m0, phi)) return rc;
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_cell_centers(const float* pos, int64_t n_p, int n, int64_t* cells) {
+  PM_INIT();
+  if (n <= 0 || n_p < 0 || (n_p && (!pos || !cells))) return fail(PM_E_INVALID, "pm_cell_centers: bad arguments");
+  PM_HIP(ref_cell_centers(pos, n_p, n, cells, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_weights(const int64_t* cells, const float* pos, int64_t n_p, float* t) {
+  PM_INIT();
+  if (n_p < 0 || (n_p && (!pos || !cells || !t))) return fail(PM_E_INVALID, "pm_weights: bad arguments");
+  PM_HIP(ref_weights(cells, pos, n_p, t, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_sweep_one_direction(const int64_t* cells, float* xd, float* vd, const float* t, const float* phi,
+                                  int64_t n_p, int n, double da, double f_a1, double a_val, int direction) {
+  PM_INIT();
+  if (direction < 0 || direction > 2) return fail(PM_E_INVALID, "direction must be 0, 1 or 2");
+  if (n <= 0 || n_p < 0 || (n_p && (!cells || !xd || !vd || !t || !phi)))
+    return fail(PM_E_INVALID, "pm_sweep_one_direction: bad arguments");
+  PM_HIP(ref_sweep(cells, xd, vd, t, phi, n_p, n, da, f_a1, a_val, direction, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_integrate(float* pos, float* vel, int64_t n_p, int n, double a_val, double f_a1, double da,
+                        const float* phi) {
+  PM_INIT();
+  if (n <= 0 || n_p < 0 || (n_p && (!pos || !vel || !phi))) return fail(PM_E_INVALID, "pm_integrate: bad arguments");
+  PM_HIP(ref_integrate(pos, vel, n_p, n, a_val, f_a1, da, phi, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_advance_time(const float* rho, float* pos, float* vel, const float* fgrid, int64_t n_p, int n,
+                           double a, double da, double h0, double omega_lambda0, double omega_k0,
+                           double omega_m0) {
+  PM_INIT();
+  if (int rc = check_grid(n)) return rc;
+  if (n_p < 0 || !rho || !fgrid || (n_p && (!pos || !vel)) || a == 0.0)
+    return fail(PM_E_INVALID, "pm_advance_time: bad arguments");
+  Workspace* w;
+  if (int rc = get_workspace(n, true, &w)) return rc;
+  if (int rc = potential_into(rho, fgrid, n, a, omega_m0, w->phi)) return rc;
+  const double fa1 = f_of_a(a + da, h0, omega_lambda0, omega_k0);  // integrate.py:12 (Q1)
+  PM_HIP(ref_integrate(pos, vel, n_p, n, a, fa1, da, w->phi, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API double pm_f(double a, double c0, double c1, double c2) { return f_of_a(a, c0, c1, c2); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// device-resident simulation
+// ------------------------------------------------------------------------------------------
+struct pm_sim {
+  Lattice L{};
+  int prec = 4;
+  unsigned flags = 0;
+  double omega_m0 = 0.31, omega_l0 = 0.69, omega_k0 = 0.0, h0 = 0.68;
+  double mass = 1.0;
+  void* s = nullptr;
+  void* v = nullptr;
+  void* rho = nullptr;
+  void* H = nullptr;  // half spectrum; the potential is written over it
+  void* Q = nullptr;
+  const float* fgrid = nullptr;  // optional caller-owned Green's function grid
+  FftTables* tab = nullptr;
+  hipStream_t stream = nullptr;
+  bool has_state = false, rho_valid = false, phi_valid = false;
+  bool timing = false;
+  struct Pending {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  double stage_ms[ST_COUNT] = {};
+  int64_t stage_calls[ST_COUNT] = {};
+};
+
+namespace {
+
+size_t sim_bytes(int n, int np_side, int prec) {
+  const size_t nn = (size_t)n, np = (size_t)np_side * np_side * np_side;
+  return 6 * np * prec + nn * nn * nn * prec /*rho*/ + nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/;
+}
+
+struct StageTimer {
+  pm_sim* s;
+  int stage;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(pm_sim* sim, int st) : s(sim), stage(st) {
+    if (s->timing) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, s->stream);
+    }
+  }
+  ~StageTimer() {
+    if (s->timing) {
+      (void)hipEventRecord(b, s->stream);
+      s->pending.push_back({stage, a, b});
+    }
+  }
+};
+
+int sim_collect_timing(pm_sim* s) {
+  if (s->pending.empty()) return PM_OK;
+  PM_HIP(hipStreamSynchronize(s->stream));
+  for (auto& p : s->pending) {
+    float ms = 0;
+    PM_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    s->stage_ms[p.stage] += ms;
+    s->stage_calls[p.stage] += 1;
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  s->pending.clear();
+  return PM_OK;
+}
+
+template <typename R> int sim_deposit_impl(pm_sim* s) {
+  StageTimer t(s, ST_DEPOSIT);
+  PM_HIP(sim_deposit<R>((const R*)s->s, (R*)s->rho, s->mass, s->L, s->stream));
+  s->rho_valid = true;
+  return PM_OK;
+}
+
+template <typename R> int sim_step_impl(pm_sim* s, double a, double da) {
+  const int n = s->L.n;
+  if (!s->rho_valid)
+    if (int rc = sim_deposit_impl<R>(s)) return rc;
+  cpx<R>* H = (cpx<R>*)s->H;
+  cpx<R>* Q = (cpx<R>*)s->Q;
+  GreenArgs g{};
+  g.coef = (-3 * s->omega_m0 / 8 / a) / ((double)n * n * n);  // potential.py:15 with 1/N^3
+  g.deconvolve = (s->flags & PM_FLAG_CIC_DECONVOLVE) ? 1 : 0;
+  if (s->fgrid) {
+    g.from_grid = 1;
+    g.fgrid = s->fgrid;
+  }
+  {
+    StageTimer t(s, ST_R2C);
+    PM_HIP((fft_r2c_x<R, R>((const R*)s->rho, H, Q, n, *s->tab, s->stream)));
+  }
+  {
+    StageTimer t(s, ST_Y_FWD);
+    PM_HIP(fft_c2c_y<R>(H, Q, n, -1, *s->tab, s->stream));
+  }
+  {
+    StageTimer t(s, ST_Z_GREEN);
+    PM_HIP(fft_z_green<R>(H, Q, n, g, *s->tab, s->stream));
+  }
+  {
+    StageTimer t(s, ST_Y_INV);
+    PM_HIP(fft_c2c_y<R>(H, Q, n, +1, *s->tab, s->stream));
+  }
+  {
+    StageTimer t(s, ST_C2R);
+    PM_HIP((fft_c2r_x<R, R>(H, Q, (R*)s->H, n, *s->tab, s->stream)));
+  }
+  s->phi_valid = true;
+  // integrate.py:12 — f(a+da, [H0, OL, OK]) (Q1) unless the fix flag is set
+  const double c0 = (s->flags & PM_FLAG_FIX_F_COSMOLOGY) ? s->omega_m0 : s->h0;
+  const double fa1 = f_of_a(a + da, c0, s->omega_l0, s->omega_k0);
+  const double kick = da * fa1;                          // integrate.py:94
+  const double drift = da / ((a + da) * (a + da)) * fa1;  // integrate.py:95
+  {
+    StageTimer t(s, ST_GKD);
+    PM_HIP(sim_gather_kick_drift<R>((R*)s->s, (R*)s->v, (const R*)s->H, kick, drift, s->L, s->stream));
+  }
+  s->rho_valid = false;
+  return sim_deposit_impl<R>(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+PM_API int pm_sim_footprint(int n_cells, int n_parts_side, int precision, size_t* bytes) {
+  if (!bytes || n_cells <= 0 || n_parts_side <= 0 || (precision != 32 && precision != 64))
+    return fail(PM_E_INVALID, "pm_sim_footprint: bad arguments");
+  *bytes = sim_bytes(n_cells, n_parts_side, precision / 8);
+  return PM_OK;
+}
+
+PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned flags, pm_sim** out) {
+  PM_INIT();
+  if (!out) return fail(PM_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (int rc = check_grid(n_cells)) return rc;
+  if (n_parts_side <= 0) return fail(PM_E_INVALID, "N_PARTS must be positive");
+  if (precision != 32 && precision != 64) return fail(PM_E_INVALID, "precision must be 32 or 64");
+  pm_sim* s = new pm_sim();
+  s->prec = precision / 8;
+  s->flags = flags;
+  s->L.n = n_cells;
+  s->L.np_side = n_parts_side;
+  s->L.blk = (n_parts_side % 4 == 0) ? 4 : 1;
+  s->L.nb = n_parts_side / s->L.blk;
+  s->L.np = (long)n_parts_side * n_parts_side * n_parts_side;
+  s->L.r = (double)n_cells / (double)n_parts_side;
+  s->mass = std::pow((double)n_cells / (double)n_parts_side, 3);  // pmesh.py:28
+  const size_t nn = (size_t)n_cells, pr = (size_t)s->prec;
+  auto cleanup = [&](int rc) {
+    pm_sim_destroy(s);
+    return rc;
+  };
+  if (int rc = get_tables(n_cells, s->prec, &s->tab)) return cleanup(rc);
+  if (hipMalloc(&s->s, 3 * (size_t)s->L.np * pr) != hipSuccess ||
+      hipMalloc(&s->v, 3 * (size_t)s->L.np * pr) != hipSuccess ||
+      hipMalloc(&s->rho, nn * nn * nn * pr) != hipSuccess || hipMalloc(&s->H, nn * nn * nn * pr) != hipSuccess ||
+      hipMalloc(&s->Q, nn * nn * 2 * pr) != hipSuccess)
+    return cleanup(fail(PM_E_NOMEM, "device allocation of %zu bytes failed", sim_bytes(n_cells, n_parts_side, s->prec)));
+  if (hipStreamCreate(&s->stream) != hipSuccess) return cleanup(fail(PM_E_HIP, "hipStreamCreate failed"));
+  *out = s;
+  return PM_OK;
+}
+
+PM_API int pm_sim_destroy(pm_sim* s) {
+  if (!s) return PM_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto& p : s->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  (void)hipFree(s->s);
+  (void)hipFree(s->v);
+  (void)hipFree(s->rho);
+  (void)hipFree(s->H);
+  (void)hipFree(s->Q);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return PM_OK;
+}
+
+PM_API int pm_sim_set_cosmology(pm_sim* s, double omega_m0, double omega_lambda0, double omega_k0, double h0) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  s->omega_m0 = omega_m0;
+  s->omega_l0 = omega_lambda0;
+  s->omega_k0 = omega_k0;
+  s->h0 = h0;
+  return PM_OK;
+}
+
+PM_API int pm_sim_set_mass(pm_sim* s, double mass) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  if (!(mass > 0.0)) return fail(PM_E_INVALID, "mass must be positive");
+  if (mass != s->mass) s->rho_valid = false;
+  s->mass = mass;
+  return PM_OK;
+}
+
+PM_API int pm_sim_set_green(pm_sim* s, const float* fgrid) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  s->fgrid = fgrid;
+  return PM_OK;
+}
+
+PM_API int pm_sim_set_density(pm_sim* s, const void* rho, int dtype_bytes, int on_device) {
+  if (!s || !rho) return fail(PM_E_INVALID, "null argument");
+  if (dtype_bytes != 4 && dtype_bytes != 8) return fail(PM_E_INVALID, "dtype_bytes must be 4 or 8");
+  const size_t cnt = (size_t)s->L.n * s->L.n * s->L.n;
+  if (dtype_bytes == s->prec) {
+    PM_HIP(hipMemcpyAsync(s->rho, rho, cnt * dtype_bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                          s->stream));
+  } else {
+    const void* src = rho;
+    void* tmp = nullptr;
+    if (!on_device) {
+      if (hipMalloc(&tmp, cnt * dtype_bytes) != hipSuccess) return fail(PM_E_NOMEM, "staging allocation failed");
+      if (hipMemcpyAsync(tmp, rho, cnt * dtype_bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(PM_E_HIP, "host to device copy failed");
+      }
+      src = tmp;
+    }
+    hipError_t e = s->prec == 4 ? convert_grid<double, float>((const double*)src, (float*)s->rho, cnt, s->stream)
+                                : convert_grid<float, double>((const float*)src, (double*)s->rho, cnt, s->stream);
+    hipError_t es = hipStreamSynchronize(s->stream);
+    if (tmp) (void)hipFree(tmp);
+    if (e != hipSuccess || es != hipSuccess) return fail(PM_E_HIP, "density upload failed");
+  }
+  PM_HIP(hipStreamSynchronize(s->stream));
+  s->rho_valid = true;
+  return PM_OK;
+}
+
+PM_API int pm_sim_set_state(pm_sim* s, const void* pos, const void* vel, int dtype_bytes, int on_device) {
+  if (!s || !pos || !vel) return fail(PM_E_INVALID, "pm_sim_set_state: null argument");
+  if (dtype_bytes != 4 && dtype_bytes != 8) return fail(PM_E_INVALID, "dtype_bytes must be 4 or 8");
+  const size_t bytes = 3 * (size_t)s->L.np * dtype_bytes;
+  const void* dp = pos;
+  const void* dv = vel;
+  void* tmp = nullptr;
+  if (!on_device) {
+    if (hipMalloc(&tmp, 2 * bytes) != hipSuccess) return fail(PM_E_NOMEM, "staging allocation failed");
+    dp = tmp;
+    dv = (char*)tmp + bytes;
+    hipError_t e1 = hipMemcpyAsync((void*)dp, pos, bytes, hipMemcpyHostToDevice, s->stream);
+    hipError_t e2 = hipMemcpyAsync((void*)dv, vel, bytes, hipMemcpyHostToDevice, s->stream);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      (void)hipFree(tmp);
+      return fail(PM_E_HIP, "host to device copy failed");
+    }
+  }
+  hipError_t e;
+  if (s->prec == 4) {
+    e = dtype_bytes == 4 ? sim_set_state<float, float>((const float*)dp, (const float*)dv, (float*)s->s, (float*)s->v,
+                                                       s->L, s->stream)
+                         : sim_set_state<float, double>((const double*)dp, (const double*)dv, (float*)s->s,
+                                                        (float*)s->v, s->L, s->stream);
+  } else {
+    e = dtype_bytes == 4 ? sim_set_state<double, float>((const float*)dp, (const float*)dv, (double*)s->s,
+                                                        (double*)s->v, s->L, s->stream)
+                         : sim_set_state<double, double>((const double*)dp, (const double*)dv, (double*)s->s,
+                                                         (double*)s->v, s->L, s->stream);
+  }
+  hipError_t es = hipStreamSynchronize(s->stream);
+  if (tmp) (void)hipFree(tmp);
+  if (e != hipSuccess || es != hipSuccess) return fail(PM_E_HIP, "set_state failed: %s", hipGetErrorString(e ? e : es));
+  s->has_state = true;
+  s->rho_valid = false;
+  s->phi_valid = false;
+  return PM_OK;
+}
+
+PM_API int pm_sim_get_state(pm_sim* s, void* pos, void* vel, int dtype_bytes, int on_device) {
+  if (!s || !pos || !vel) return fail(PM_E_INVALID, "pm_sim_get_state: null argument");
+  if (!s->has_state) return fail(PM_E_STATE, "state was never set");
+  if (dtype_bytes != 4 && dtype_bytes != 8) return fail(PM_E_INVALID, "dtype_bytes must be 4 or 8");
+  const size_t bytes = 3 * (size_t)s->L.np * dtype_bytes;
+  void* dp = pos;
+  void* dv = vel;
+  void* tmp = nullptr;
+  if (!on_device) {
+    if (hipMalloc(&tmp, 2 * bytes) != hipSuccess) return fail(PM_E_NOMEM, "staging allocation failed");
+    dp = tmp;
+    dv = (char*)tmp + bytes;
+  }
+  hipError_t e;
+  if (s->prec == 4) {
+    e = dtype_bytes == 4 ? sim_get_state<float, float>((const float*)s->s, (const float*)s->v, (float*)dp, (float*)dv,
+                                                       s->L, s->stream)
+                         : sim_get_state<float, double>((const float*)s->s, (const float*)s->v, (double*)dp,
+                                                        (double*)dv, s->L, s->stream);
+  } else {
+    e = dtype_bytes == 4 ? sim_get_state<double, float>((const double*)s->s, (const double*)s->v, (float*)dp,
+                                                        (float*)dv, s->L, s->stream)
+                         : sim_get_state<double, double>((const double*)s->s, (const double*)s->v, (double*)dp,
+                                                         (double*)dv, s->L, s->stream);
+  }
+  if (e == hipSuccess && !on_device) {
+    e = hipMemcpyAsync(pos, dp, bytes, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(vel, dv, bytes, hipMemcpyDeviceToHost, s->stream);
+  }
+  hipError_t es = hipStreamSynchronize(s->stream);
+  if (tmp) (void)hipFree(tmp);
+  if (e != hipSuccess || es != hipSuccess) return fail(PM_E_HIP, "get_state failed: %s", hipGetErrorString(e ? e : es));
+  return PM_OK;
+}
+
+PM_API int pm_sim_deposit(pm_sim* s) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  if (!s->has_state) return fail(PM_E_STATE, "state was never set");
+  int rc = s->prec == 4 ? sim_deposit_impl<float>(s) : sim_deposit_impl<double>(s);
+  if (rc) return rc;
+  PM_HIP(hipStreamSynchronize(s->stream));
+  return PM_OK;
+}
+
+static int copy_grid_out(pm_sim* s, const void* src, void* dst, int dtype_bytes, int on_device) {
+  if (dtype_bytes != 4 && dtype_bytes != 8) return fail(PM_E_INVALID, "dtype_bytes must be 4 or 8");
+  const size_t cnt = (size_t)s->L.n * s->L.n * s->L.n;
+  void* d = dst;
+  void* tmp = nullptr;
+  if (dtype_bytes == s->prec) {
+    if (on_device) PM_HIP(hipMemcpyAsync(dst, src, cnt * dtype_bytes, hipMemcpyDeviceToDevice, s->stream));
+    else PM_HIP(hipMemcpyAsync(dst, src, cnt * dtype_bytes, hipMemcpyDeviceToHost, s->stream));
+    PM_HIP(hipStreamSynchronize(s->stream));
+    return PM_OK;
+  }
+  if (!on_device) {
+    if (hipMalloc(&tmp, cnt * dtype_bytes) != hipSuccess) return fail(PM_E_NOMEM, "staging allocation failed");
+    d = tmp;
+  }
+  hipError_t e = s->prec == 4 ? convert_grid<float, double>((const float*)src, (double*)d, cnt, s->stream)
+                              : convert_grid<double, float>((const double*)src, (float*)d, cnt, s->stream);
+  if (e == hipSuccess && !on_device) e = hipMemcpyAsync(dst, d, cnt * dtype_bytes, hipMemcpyDeviceToHost, s->stream);
+  hipError_t es = hipStreamSynchronize(s->stream);
+  if (tmp) (void)hipFree(tmp);
+  if (e != hipSuccess || es != hipSuccess) return fail(PM_E_HIP, "grid copy failed");
+  return PM_OK;
+}
+
+PM_API int pm_sim_get_density(pm_sim* s, void* rho, int dtype_bytes, int on_device) {
+  if (!s || !rho) return fail(PM_E_INVALID, "null argument");
+  if (!s->has_state) return fail(PM_E_STATE, "state was never set");
+  if (!s->rho_valid) {
+    int rc = s->prec == 4 ? sim_deposit_impl<float>(s) : sim_deposit_impl<double>(s);
+    if (rc) return rc;
+  }
+  return copy_grid_out(s, s->rho, rho, dtype_bytes, on_device);
+}
+
+PM_API int pm_sim_get_potential(pm_sim* s, void* phi, int dtype_bytes, int on_device) {
+  if (!s || !phi) return fail(PM_E_INVALID, "null argument");
+  if (!s->phi_valid) return fail(PM_E_STATE, "no potential computed yet (run a step first)");
+  return copy_grid_out(s, s->H, phi, dtype_bytes, on_device);
+}
+
+PM_API int pm_sim_step(pm_sim* s, double a, double da) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  if (!s->has_state) return fail(PM_E_STATE, "state was never set");
+  if (!(a > 0.0) || !(a + da > 0.0)) return fail(PM_E_INVALID, "scale factor must be positive");
+  int rc = s->prec == 4 ? sim_step_impl<float>(s, a, da) : sim_step_impl<double>(s, a, da);
+  if (rc) return rc;
+  PM_HIP(hipStreamSynchronize(s->stream));
+  return PM_OK;
+}
+
+PM_API int pm_sim_run(pm_sim* s, double a, double da, int n_steps, double* a_out) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  if (!s->has_state) return fail(PM_E_STATE, "state was never set");
+  if (n_steps < 0) return fail(PM_E_INVALID, "n_steps must be >= 0");
+  for (int k = 0; k < n_steps; ++k) {
+    if (!(a > 0.0)) return fail(PM_E_INVALID, "scale factor must be positive");
+    int rc = s->prec == 4 ? sim_step_impl<float>(s, a, da) : sim_step_impl<double>(s, a, da);
+    if (rc) return rc;
+    a += da;  // pmesh.py:63
+  }
+  PM_HIP(hipStreamSynchronize(s->stream));
+  if (a_out) *a_out = a;
+  return PM_OK;
+}
+
+PM_API int pm_sim_set_timing(pm_sim* s, int enable) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  if (int rc = sim_collect_timing(s)) return rc;
+  s->timing = enable != 0;
+  for (int i = 0; i < ST_COUNT; ++i) {
+    s->stage_ms[i] = 0;
+    s->stage_calls[i] = 0;
+  }
+  return PM_OK;
+}
+
+PM_API int pm_sim_get_timing(pm_sim* s, double* ms, int64_t* calls, int n_stages) {
+  if (!s || !ms || !calls) return fail(PM_E_INVALID, "null argument");
+  if (int rc = sim_collect_timing(s)) return rc;
+  for (int i = 0; i < n_stages && i < ST_COUNT; ++i) {
+    ms[i] = s->stage_ms[i];
+    calls[i] = s->stage_calls[i];
+  }
+  return PM_OK;
+}
+
+PM_API int pm_sim_sync(pm_sim* s) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  PM_HIP(hipStreamSynchronize(s->stream));
+  return PM_OK;
+}
+
+}  // extern "C"

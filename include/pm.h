@@ -1,0 +1,133 @@
+/*
+ * libpm — particle-mesh (PM) step for AMD Instinct MI355X (gfx950): C ABI.
+ *
+ * Every entry point returns an int status: 0 (PM_OK) on success, otherwise one of the
+ * PM_E_* codes below; pm_last_error() then returns a thread-local message.  No C++
+ * exception crosses this boundary.  Pointers named d_* are device pointers (HBM); h_*
+ * are host pointers.  Layouts are the reference's NumPy layouts (C order):
+ *   positions / velocities : float32 (3, N_p)   row 0 = x (grid axis 2), row 2 = z (axis 0)
+ *   grids                  : float32 (N, N, N)  indexed [z, y, x]
+ *   spectra                : complex128 (N, N, N) (interleaved re, im doubles)
+ *   weights t              : float32 (8, N_p); cell centers int64 (3, N_p)
+ * File:line citations refer to the reference's src/ directory.
+ */
+#ifndef PM_H
+#define PM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define PM_API __attribute__((visibility("default")))
+#else
+#define PM_API
+#endif
+
+#define PM_OK 0
+#define PM_E_INVALID 1
+#define PM_E_HIP 2
+#define PM_E_NOMEM 3
+#define PM_E_STATE 4
+#define PM_E_UNSUPPORTED 5
+
+/* sim flags */
+#define PM_FLAG_CIC_DECONVOLVE 1u   /* divide by the CIC window of deposit and gather        */
+#define PM_FLAG_FIX_F_COSMOLOGY 2u  /* f() gets [Omega_m, OL, OK] instead of [H0, OL, OK]     */
+
+/* ---------------------------------------------------------------- library / device memory */
+PM_API int pm_init(int device);                 /* select device, create the library stream */
+PM_API const char* pm_last_error(void);         /* thread-local message of the last failure */
+PM_API int pm_version(void);
+PM_API int pm_supported_grid(int n_cells);      /* 1 if N is a supported FFT size            */
+PM_API int pm_malloc(void** d_ptr, size_t bytes);
+PM_API int pm_free(void* d_ptr);
+PM_API int pm_memcpy_htod(void* d_dst, const void* h_src, size_t bytes);
+PM_API int pm_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
+PM_API int pm_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+PM_API int pm_memset(void* d_dst, int value, size_t bytes);
+PM_API int pm_synchronize(void);
+
+/* ------------------------------------------- reference-function mirrors (device pointers) */
+/* density.py:7-48  density(positions, mass): CIC deposit into a new float32 N^3 grid.       */
+PM_API int pm_density(const float* d_positions, int64_t n_p, int n_cells, double mass,
+                      float* d_rho);
+/* fourier_utils.py:5-16  fourier_grid(): 1/sum sin^2(k/2), G(0) = 0.                         */
+PM_API int pm_fourier_grid(int n_cells, float* d_fgrid);
+/* potential.py:17-21  density_k(density): forward c2c DFT of the real grid (complex128).    */
+PM_API int pm_density_k(const float* d_rho, int n_cells, void* d_rho_k);
+/* potential.py:12-15  potential_k(density_k, fgrid, a) = -3*Omega_m/8/a * fgrid * density_k */
+PM_API int pm_potential_k(const void* d_rho_k, const float* d_fgrid, int n_cells, double a,
+                          double omega_m0, void* d_phi_k);
+/* potential.py:23-29  potential_real(potential_k): Re(IDFT)/N^3 as float32.                 */
+PM_API int pm_potential_real(const void* d_phi_k, int n_cells, float* d_phi);
+/* potential.py:7-10  potential(density, fgrid, a), fused (complex128 arithmetic).           */
+PM_API int pm_potential(const float* d_rho, const float* d_fgrid, int n_cells, double a,
+                        double omega_m0, float* d_phi);
+/* integrate.py:20  cell_centers = floor(positions).astype(int64) % N_CELLS                  */
+PM_API int pm_cell_centers(const float* d_positions, int64_t n_p, int n_cells,
+                           int64_t* d_cells);
+/* integrate.py:27-52  weights(cell_centers, positions) -> float32 (8, N_p)                  */
+PM_API int pm_weights(const int64_t* d_cells, const float* d_positions, int64_t n_p,
+                      float* d_t);
+/* integrate.py:54-97  sweep_one_direction(...): kick + drift of one axis, in place.         */
+PM_API int pm_sweep_one_direction(const int64_t* d_cells, float* d_positions_dir,
+                                  float* d_velocities_dir, const float* d_t,
+                                  const float* d_phi, int64_t n_p, int n_cells, double da,
+                                  double f_a1, double a_val, int direction);
+/* integrate.py:15-25  integrate(positions, velocities, a, f_a1, da, potentials), in place.  */
+PM_API int pm_integrate(float* d_positions, float* d_velocities, int64_t n_p, int n_cells,
+                        double a_val, double f_a1, double da, const float* d_phi);
+/* integrate.py:9-13  advance_time(density, positions, velocities, fgrid, a, da), in place.  */
+PM_API int pm_advance_time(const float* d_rho, float* d_positions, float* d_velocities,
+                           const float* d_fgrid, int64_t n_p, int n_cells, double a, double da,
+                           double h0, double omega_lambda0, double omega_k0, double omega_m0);
+/* cosmology.py:20-27  f(a, [c0, c1, c2]) = 1/sqrt((c0 + c2*a + c1*a^3)/a)  (host scalar)    */
+PM_API double pm_f(double a, double c0, double c1, double c2);
+
+/* ------------------------------------- device-resident simulation (fused PM step, pmesh.py) */
+typedef struct pm_sim pm_sim;
+/* precision: 32 (FP32 state, grids and FFT) or 64 (FP64 throughout).                        */
+PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned flags,
+                         pm_sim** out);
+PM_API int pm_sim_destroy(pm_sim* sim);
+/* configure_me.py:34-38 constants the step reads                                             */
+PM_API int pm_sim_set_cosmology(pm_sim* sim, double omega_m0, double omega_lambda0,
+                                double omega_k0, double h0);
+/* density.py:8 `mass` argument (default (N_CELLS/N_PARTS)^3, pmesh.py:28)                   */
+PM_API int pm_sim_set_mass(pm_sim* sim, double mass);
+/* potential.py:15 `fgrid`: use a caller-owned float32 N^3 device grid instead of the
+ * analytic Green's function (NULL restores the analytic one).  The grid must outlive use.   */
+PM_API int pm_sim_set_green(pm_sim* sim, const float* d_fgrid);
+/* positions/velocities in the reference layout (3, N_p), absolute positions in [0, N).
+ * dtype_bytes: 4 (float32) or 8 (float64); on_device: 0 host pointers, 1 device pointers.   */
+PM_API int pm_sim_set_state(pm_sim* sim, const void* positions, const void* velocities,
+                            int dtype_bytes, int on_device);
+PM_API int pm_sim_get_state(pm_sim* sim, void* positions, void* velocities, int dtype_bytes,
+                            int on_device);
+/* density.py:7-48 on the resident state (mass = (N/N_PARTS)^3, pmesh.py:28)                 */
+PM_API int pm_sim_deposit(pm_sim* sim);
+/* replace the density the next step solves for (integrate.py:9 `density` argument)         */
+PM_API int pm_sim_set_density(pm_sim* sim, const void* rho, int dtype_bytes, int on_device);
+/* copy the current density / potential grid out (float32 or float64, host or device)       */
+PM_API int pm_sim_get_density(pm_sim* sim, void* rho, int dtype_bytes, int on_device);
+PM_API int pm_sim_get_potential(pm_sim* sim, void* phi, int dtype_bytes, int on_device);
+/* one PM step: pmesh.py:60-61 (density + advance_time) at scale factor a with step da.     */
+PM_API int pm_sim_step(pm_sim* sim, double a, double da);
+/* n_steps steps from a with step da, enqueued back to back; returns the final a.            */
+PM_API int pm_sim_run(pm_sim* sim, double a, double da, int n_steps, double* a_out);
+/* per-stage timing with HIP events on the sim's stream (0 off, 1 on); stage order:
+ * deposit, r2c_x, c2c_y_fwd, z_green, c2c_y_inv, c2r_x, gather_kick_drift                   */
+PM_API int pm_sim_set_timing(pm_sim* sim, int enable);
+PM_API int pm_sim_get_timing(pm_sim* sim, double* stage_ms, int64_t* stage_calls, int n_stages);
+PM_API int pm_sim_sync(pm_sim* sim);
+/* bytes of device memory held by the sim                                                     */
+PM_API int pm_sim_footprint(int n_cells, int n_parts_side, int precision, size_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PM_H */

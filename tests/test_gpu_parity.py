@@ -1,0 +1,270 @@
+"""GPU parity: every HIP entry point against the float64 oracle on the same seeded inputs.
+
+Tolerances (norm-wise relative error, fields cross zero):
+  complex128 reference-layout path (density_k, potential*, advance_time): 1e-6 for float32
+    outputs (one float32 rounding of each output), 1e-12 for complex128 spectra;
+  fused FP32 step: rho, phi, accelerations <= 1e-5; FP64 step <= 1e-10;
+  cell assignment: bit-exact on equal positions.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pm_oracle as O
+from helpers import make_ics, periodic_diff, rel, uniform_positions
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(16, 8), (32, 16), (48, 24), (64, 32), (96, 48), (128, 64)]
+
+
+@pytest.fixture(scope="module")
+def ops():
+    from pmgpu import ops as _ops
+    return _ops
+
+
+# ------------------------------------------------------------------ reference-layout entry points
+@pytest.mark.parametrize("n", [8, 16, 32, 48, 64, 96, 192])
+def test_density_k_matches_dft(ops, n):
+    rho = np.random.default_rng(n).uniform(0, 2, (n, n, n)).astype(np.float32)
+    got = ops.density_k(rho)
+    ref = np.fft.fftn(rho.astype(np.float64))
+    print(f"N={n} density_k rel err {rel(got, ref):.2e}")
+    assert got.dtype == np.complex128 and got.shape == (n, n, n)
+    assert rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [8, 32, 96])
+def test_potential_real_of_general_spectrum(ops, n):
+    r = np.random.default_rng(n)
+    x = r.normal(size=(n, n, n)) + 1j * r.normal(size=(n, n, n))
+    got = ops.potential_real(x)
+    ref = np.fft.ifftn(x).real
+    assert got.dtype == np.float32
+    assert rel(got, ref) < 1e-6
+
+
+@pytest.mark.parametrize("n,npart", SIZES)
+def test_density_matches_oracle(ops, n, npart):
+    pos, _ = make_ics(n, npart, seed=n)
+    got = ops.density(pos, (n / npart) ** 3, n)
+    ref = O.density(pos, (n / npart) ** 3, n)
+    print(f"N={n} density rel err {rel(got, ref):.2e}")
+    assert got.dtype == np.float32 and got.shape == (n, n, n)
+    assert rel(got, ref) < 1e-6
+    assert abs(got.sum(dtype=np.float64) - npart ** 3 * (n / npart) ** 3) < 1e-3 * n ** 3
+
+
+def test_density_edge_cases(ops):
+    n = 16
+    # empty particle set
+    got = ops.density(np.zeros((3, 0), np.float32), 1.0, n)
+    assert got.shape == (n, n, n) and not got.any()
+    # Q4: a float32 position equal to N is the particle at 0; N - eps wraps its +1 corner
+    pos = np.array([[16.0, 15.75], [0.0, 3.0], [0.0, 15.0]], np.float32)
+    got = ops.density(pos, 1.0, n)
+    np.testing.assert_allclose(got, O.density(pos, 1.0, n), atol=1e-7)
+
+
+def test_fourier_grid_matches(ops):
+    n = 32
+    got = ops.fourier_grid(n)
+    # float32 sin of float32 k: NumPy and the device may differ by a couple of ulps
+    np.testing.assert_allclose(got, O.fourier_grid(n, ref_dtypes=True), rtol=5e-7, atol=0)
+    np.testing.assert_allclose(got, O.fourier_grid(n), rtol=1e-6, atol=0)
+    assert got[0, 0, 0] == 0.0
+
+
+@pytest.mark.parametrize("n,npart", [(32, 16), (48, 24), (64, 32)])
+def test_potential_chain_matches_oracle(ops, n, npart):
+    pos, _ = make_ics(n, npart, seed=3)
+    rho = O.density(pos, (n / npart) ** 3, n).astype(np.float32)
+    g = O.fourier_grid(n, ref_dtypes=True)
+    a, om = 0.05, 0.3
+    ref = O.potential(rho, g, a, om)
+    fused = ops.potential(rho, g, a, om)
+    rk = ops.density_k(rho)
+    pk = ops.potential_k(rk, g, a, om)
+    chain = ops.potential_real(pk)
+    np.testing.assert_allclose(pk, O.potential_k(O.density_k(rho), g, a, om), rtol=1e-10,
+                               atol=1e-10 * np.abs(pk).max())
+    print(f"N={n} potential fused {rel(fused, ref):.2e} chain {rel(chain, ref):.2e}")
+    assert rel(fused, ref) < 1e-6 and rel(chain, ref) < 1e-6
+
+
+@pytest.mark.parametrize("n,npart", [(32, 16), (64, 32)])
+def test_cells_weights_bit_exact(ops, n, npart):
+    pos, _ = make_ics(n, npart, seed=5)
+    # 0, the largest float32 below N, N itself (Q4: wraps to 0) and an integer
+    pos[0, :4] = [0.0, np.nextafter(np.float32(n), np.float32(0)), float(n), 7.0]
+    cells = ops.cell_centers(pos, n)
+    ref_cells = O.cell_centers(pos, n)
+    np.testing.assert_array_equal(cells, ref_cells)
+    assert cells[0, 1] == n - 1 and cells[0, 2] == 0
+    t = ops.weights(cells, pos)
+    np.testing.assert_allclose(t, O.weights(ref_cells, O.wrap_positions(pos, n)), rtol=1e-6,
+                               atol=1e-7)
+
+
+@pytest.mark.parametrize("n,npart", [(32, 16), (64, 32)])
+def test_sweep_and_integrate_match_oracle(ops, n, npart):
+    pos, vel = make_ics(n, npart, seed=7)
+    rho = O.density(pos, (n / npart) ** 3, n)
+    phi = O.potential(rho, O.fourier_grid(n), 0.05, 0.3).astype(np.float32)
+    a, da, fa1 = 0.05, 0.01, 0.9
+    rx, rv = O.integrate(pos, vel, a, fa1, da, phi, n)
+    # integrate (all axes fused)
+    x, v = pos.copy(), vel.copy()
+    ops.integrate(x, v, a, fa1, da, phi, n)
+    assert rel(v - vel, rv - vel) < 1e-5
+    assert np.abs(periodic_diff(x, rx, n)).max() < 4e-6 * n
+    # the three single-axis sweeps with the reference's precomputed cells and weights
+    cells = ops.cell_centers(pos, n)
+    t = ops.weights(cells, pos)
+    x2, v2 = pos.copy(), vel.copy()
+    for d in range(3):
+        xd, vd = x2[d].copy(), v2[d].copy()
+        ops.sweep_one_direction(cells, xd, vd, t, phi, da, fa1, a, d, n)
+        x2[d], v2[d] = xd, vd
+    assert rel(v2 - vel, rv - vel) < 1e-5
+    assert np.abs(periodic_diff(x2, rx, n)).max() < 4e-6 * n
+
+
+@pytest.mark.parametrize("n,npart", [(32, 16), (64, 32)])
+def test_advance_time_stateless(ops, n, npart):
+    pos, vel = make_ics(n, npart, seed=9)
+    p = O.OracleParams(n_cells=n, n_parts=npart, omega_m0=0.3, omega_lambda0=0.7, h0=0.7)
+    rho = O.density(pos, p.mass, n).astype(np.float32)
+    g = O.fourier_grid(n, ref_dtypes=True)
+    a, da = 0.02, 0.0098
+    rx, rv = O.advance_time(rho, pos, vel, g, a, da, p)
+    x, v = pos.copy(), vel.copy()
+    out = ops.advance_time(rho, x, v, g, a, da, [0.7, 0.7, 0.0], 0.3)
+    assert out[0] is x and out[1] is v            # in place, like the reference
+    assert rel(v - vel, rv - vel) < 1e-5
+    assert np.abs(periodic_diff(x, rx, n)).max() < 4e-6 * n
+
+
+# ------------------------------------------------------------------------- fused resident sim
+def _sim(n, npart, prec, **kw):
+    from pmgpu.sim import PMSim
+    return PMSim(n, npart, prec, omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7, **kw)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("n,npart", SIZES)
+def test_sim_one_step_matches_oracle(n, npart, prec):
+    pos, vel = make_ics(n, npart, seed=11)
+    sim = _sim(n, npart, prec)
+    sim.set_state(pos, vel)
+    x0, v0 = sim.get_state(np.float64 if prec == 64 else np.float32)
+    # the displacement round trip loses at most half an ulp of the stored displacement
+    s_max = float(np.abs(periodic_diff(pos, O.lattice_q(npart, n), n)).max())
+    ulp = float(np.spacing(np.float32(s_max))) if prec == 32 else 0.0
+    assert np.abs(periodic_diff(x0, pos, n)).max() <= ulp / 2
+    np.testing.assert_array_equal(np.asarray(v0, np.float32), vel)
+    p = O.OracleParams(n_cells=n, n_parts=npart, omega_m0=0.3, omega_lambda0=0.7, h0=0.7)
+    rho_ref = O.density(x0, p.mass, n)
+    rho = sim.density(np.float64)
+    a, da = 0.02, 0.0098
+    x1r, v1r, phi_ref = O.advance_time(rho_ref, x0, v0, O.fourier_grid(n), a, da, p,
+                                       return_phi=True)
+    sim.step(a, da)
+    phi = sim.potential(np.float64)
+    x1, v1 = sim.get_state(np.float64)
+    tol = 1e-5 if prec == 32 else 1e-10
+    e_rho, e_phi = rel(rho, rho_ref), rel(phi, phi_ref)
+    e_acc = rel(v1 - v0, v1r - v0)
+    e_dx = rel(periodic_diff(x1, x0, n), periodic_diff(x1r, x0, n))
+    print(f"N={n} fp{prec}: rho {e_rho:.2e} phi {e_phi:.2e} accel {e_acc:.2e} drift {e_dx:.2e}")
+    assert e_rho < tol and e_phi < tol and e_acc < tol
+    assert e_dx < (1e-4 if prec == 32 else 1e-9)
+    sim.close()
+
+
+def test_sim_cell_assignment_bit_exact():
+    n, npart = 64, 32
+    pos, vel = make_ics(n, npart, seed=13)
+    sim = _sim(n, npart, 32)
+    sim.set_state(pos, vel)
+    for _ in range(3):
+        sim.step(0.05, 0.01)
+    x, _ = sim.get_state(np.float32)
+    # the sim deposits from its own canonical float32 positions: the oracle's deposit of the
+    # materialised positions must then see identical cells (weights differ only by rounding)
+    rho = sim.density(np.float64)
+    ref = O.density(x, sim.n_cells ** 3 / sim.n_p, n)
+    assert rel(rho, ref) < 1e-6
+    sim.close()
+
+
+def test_sim_fp32_vs_fp64_multistep_pk():
+    n, npart = 64, 32
+    pos, vel = make_ics(n, npart, seed=17, box=64.0)
+    sims = [_sim(n, npart, 32), _sim(n, npart, 64)]
+    for s in sims:
+        s.set_state(pos, vel)
+    a0, da = 0.02, 0.0098
+    for s in sims:
+        s.run(a0, da, 100)
+    r32, r64 = (s.density(np.float64) for s in sims)
+    k, p32, _ = O.power_spectrum(r32, 64.0)
+    _, p64, _ = O.power_spectrum(r64, 64.0)
+    sel = np.arange(1, len(k) + 1) <= n // 4   # k < k_Nyquist / 2
+    err = np.max(np.abs(p32[sel] / p64[sel] - 1))
+    print(f"P(k) FP32 vs FP64 after 100 steps: max rel err {err:.2e}")
+    assert err < 5e-3
+    for s in sims:
+        s.close()
+
+
+def test_sim_run_equals_repeated_step():
+    n, npart = 32, 16
+    pos, vel = make_ics(n, npart, seed=19)
+    s1, s2 = _sim(n, npart, 32), _sim(n, npart, 32)
+    s1.set_state(pos, vel)
+    s2.set_state(pos, vel)
+    a, da = 0.02, 0.0098
+    a_end = s1.run(a, da, 5)
+    for k in range(5):
+        s2.step(a, da)
+        a += da
+    assert a_end == a
+    x1, v1 = s1.get_state()
+    x2, v2 = s2.get_state()
+    np.testing.assert_allclose(x1, x2, atol=1e-5)
+    np.testing.assert_allclose(v1, v2, rtol=1e-5, atol=1e-6 * np.abs(v1).max())
+
+
+def test_sim_deconvolve_and_fix_flags_change_result():
+    n, npart = 32, 16
+    pos, vel = make_ics(n, npart, seed=21)
+    base = _sim(n, npart, 64)
+    dec = _sim(n, npart, 64, cic_deconvolve=True)
+    fix = _sim(n, npart, 64, fix_f_cosmology=True)
+    outs = []
+    for s in (base, dec, fix):
+        s.set_state(pos, vel)
+        s.step(0.02, 0.01)
+        outs.append(s.get_state(np.float64)[1])
+    assert rel(outs[1] - vel, outs[0] - vel) > 1e-3        # deconvolution boosts small scales
+    # FIX_F uses f(a+da, [Om, OL, OK]) instead of [H0, OL, OK]: the kick scales by the ratio
+    a1 = 0.03
+    ratio = O.f(a1, [0.3, 0.7, 0.0]) / O.f(a1, [0.7, 0.7, 0.0])
+    assert rel(outs[2] - vel, ratio * (outs[0] - vel)) < 1e-9
+
+
+def test_sim_rejects_bad_input():
+    from pmgpu.sim import PMSim
+    with pytest.raises(RuntimeError, match="supported grid"):
+        PMSim(100, 50, 32)
+    with pytest.raises(RuntimeError):
+        PMSim(64, 32, 16)
+    s = _sim(32, 16, 32)
+    with pytest.raises(RuntimeError, match="never set"):
+        s.step(0.1, 0.01)
+    with pytest.raises(ValueError):
+        s.set_state(np.zeros((3, 10), np.float32), np.zeros((3, 10), np.float32))
+    s.close()
