@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark of the PM step on one MI355X.
+
+Workload: 256^3 particles on a 512^3 grid (the reference README's shape), box 250 Mpc/h,
+Omega_m 0.3, Omega_L 0.7, h 0.7, Eisenstein-Hu no-wiggle P(k), n_s 0.96, sigma_8 0.8, seed 7,
+Zel'dovich ICs at a = 0.02, 1000 steps to a = 1 (pmesh.py loop).  One "step" is one PM step
+(deposit, FFT Poisson solve, gather + kick + drift) of the device-resident simulation.
+
+The timed region is K consecutive steps starting from the initial conditions (K = 1000 by
+default: the whole run), FP32; the same run is then timed in FP64.  `value` is
+particle-updates/s = N_p * K / wall time, inputs resident in HBM.  A second, shorter FP32 pass
+records HIP events around every stage on the simulation's stream for the per-kernel roofline.
+The CPU baseline is the C/OpenMP port of the reference loops plus scipy complex128 FFTs,
+timed on this host for a bounded number of steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "cosmological-particle-mesh-simulation_amd"
+sys.path.insert(0, str(PKG))
+
+METRIC = "PM particle-updates/s at 256³/512³ FP32 vs FP64 GPU; serve p50/p99"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = dict(n_cells=512, n_parts=256, box=250.0, seed=7, a_init=0.02, a_end=1.0, steps=1000,
+                om=0.3, ol=0.7, ok=0.0, h=0.7, ob=0.045, ns=0.96, sigma8=0.8)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stage_bytes(n, n_p, prec):
+    """Algorithmic HBM bytes per launch of each stage (one read + one write of what it must
+    touch).  Half spectrum: N^2 (N/2+1) complex values."""
+    r = prec // 8
+    cells = n ** 3
+    half = n * n * (n // 2 + 1) * 2 * r
+    return {
+        "deposit": 3 * r * n_p + r * cells,            # read displacements, write rho
+        "r2c_x": r * cells + half,                      # read rho, write half spectrum
+        "c2c_y_fwd": 2 * half,
+        "z_green": 2 * half,
+        "c2c_y_inv": 2 * half,
+        "c2r_x": half + r * cells,                      # read half spectrum, write phi
+        "gather_kick_drift": 12 * r * n_p + r * cells,  # read/write x, v; read phi
+    }
+
+
+def step_bytes(n, n_p, prec):
+    """Per-step algorithmic bytes as defined for this metric: 60 N_p + 32 N^3 + 16 N^2 (N/2+1)
+    at FP32 (doubled at FP64)."""
+    return (60 * n_p + 32 * n ** 3 + 16 * n * n * (n // 2 + 1)) * (prec // 32)
+
+
+def make_ics(w):
+    import initial_conditions as ic
+    p = dict(n_parts=w["n_parts"], n_cells=w["n_cells"], box=w["box"], seed=w["seed"], om=w["om"],
+             ob=w["ob"], ol=w["ol"], ok=w["ok"], h=w["h"], ns=w["ns"], sigma8=w["sigma8"],
+             a_init=w["a_init"])
+    delta = ic.gaussian_random_field(p)
+    return ic.zeldovich_numpy(delta, p)
+
+
+def run_gpu(prec, pos, vel, w, steps, warmup):
+    from pmgpu.sim import PMSim
+    sim = PMSim(w["n_cells"], w["n_parts"], prec, omega_m0=w["om"], omega_lambda0=w["ol"],
+                omega_k0=w["ok"], h0=w["h"])
+    da = (w["a_end"] - w["a_init"]) / w["steps"]
+    sim.set_state(pos, vel)
+    if warmup:
+        sim.run(w["a_init"], da, warmup)
+    sim.set_state(pos, vel)          # the timed region starts from the initial conditions
+    sim.deposit()
+    sim.sync()
+    t0 = time.perf_counter()
+    a_end = sim.run(w["a_init"], da, steps)
+    sim.sync()
+    dt = time.perf_counter() - t0
+    return sim, dt, a_end
+
+
+def stage_timing(pos, vel, w, steps):
+    from pmgpu.sim import PMSim
+    sim = PMSim(w["n_cells"], w["n_parts"], 32, omega_m0=w["om"], omega_lambda0=w["ol"],
+                omega_k0=w["ok"], h0=w["h"])
+    da = (w["a_end"] - w["a_init"]) / w["steps"]
+    sim.set_state(pos, vel)
+    sim.run(w["a_init"], da, 2)
+    sim.set_timing(True)
+    sim.run(w["a_init"] + 2 * da, da, steps)
+    t = sim.timing()
+    sim.close()
+    return {k: ms / max(c, 1) for k, (ms, c) in t.items()}, {k: c for k, (ms, c) in t.items()}
+
+
+def pk_error(r32, r64, box):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pm_oracle as O
+    k, p32, _ = O.power_spectrum(r32, box)
+    _, p64, _ = O.power_spectrum(r64, box)
+    n = r32.shape[0]
+    sel = np.arange(1, len(k) + 1) <= n // 4          # k < k_Nyquist / 2
+    return float(np.max(np.abs(p32[sel] / p64[sel] - 1)))
+
+
+def cpu_baseline(pos, vel, w, max_steps, budget_s):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pm_oracle as O
+    import pm_port
+    lib = pm_port.load()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or pm_port.cores()
+    threads = max(1, min(threads, pm_port.cores()))
+    lib.port_set_threads(threads)
+    n = w["n_cells"]
+    fgrid = O.fourier_grid(n, ref_dtypes=True)
+    mass = (n / w["n_parts"]) ** 3
+    da = (w["a_end"] - w["a_init"]) / w["steps"]
+    x, v = pos.copy(), vel.copy()
+    a = w["a_init"]
+    stages = {}
+    done = 0
+    t0 = time.perf_counter()
+    while done < max_steps:
+        pm_port.step(x, v, fgrid, mass, a, da, w["om"], w["ol"], w["ok"], w["h"], threads, stages)
+        a += da
+        done += 1
+        log(f"cpu baseline step {done}: {time.perf_counter() - t0:.1f} s")
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    n_p = w["n_parts"] ** 3
+    return {"value": n_p * done / dt, "unit": "particle-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{done} full PM steps of the same 256^3/512^3 workload from the same ICs "
+                      f"(C/OpenMP port of density.py/integrate.py + scipy.fft complex128 c2c), "
+                      f"{dt:.1f} s",
+            "stage_s_per_step": {k: s / done for k, s in stages.items()}}
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, if committed."""
+    f = ROOT / "profiles" / "hbm_traffic_512.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get(kernel)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=WORKLOAD["steps"])
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--timing-steps", type=int, default=50)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=25.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); a multi-GPU PM step "
+              "is not part of this build", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOAD
+    n, n_p = w["n_cells"], w["n_parts"] ** 3
+    log("initial conditions (CPU, seeded)")
+    pos, vel = make_ics(w)
+
+    log(f"FP32 GPU: {args.warmup} warm-up + {args.steps} timed steps")
+    sim32, dt32, a_end = run_gpu(32, pos, vel, w, args.steps, args.warmup)
+    log(f"FP32: {dt32:.3f} s, a_end {a_end:.4f}")
+    fp64_steps = args.fp64_steps or args.steps
+    log(f"FP64 GPU: {args.warmup} warm-up + {fp64_steps} timed steps")
+    sim64, dt64, _ = run_gpu(64, pos, vel, w, fp64_steps, args.warmup)
+    log(f"FP64: {dt64:.3f} s")
+    pk_err = None
+    if fp64_steps == args.steps:
+        pk_err = pk_error(sim32.density(np.float64), sim64.density(np.float64), w["box"])
+    sim32.close()
+    sim64.close()
+
+    log(f"per-stage HIP-event timing over {args.timing_steps} FP32 steps")
+    stage_ms, stage_calls = stage_timing(pos, vel, w, args.timing_steps)
+    sb = stage_bytes(n, n_p, 32)
+    dom = max(stage_ms, key=lambda k: stage_ms[k])
+    ach = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    v32 = n_p * args.steps / dt32
+    v64 = n_p * fp64_steps / dt64
+    ms32 = dt32 / args.steps * 1e3
+    out = {
+        "metric": METRIC,
+        "value": v32,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pm_step_256p_512g: 256^3 particles, 512^3 grid, box 250 Mpc/h, "
+                               "seed 7, EH no-wiggle sigma8 0.8, a 0.02 -> 1",
+                   "n_parts": w["n_parts"], "n_cells": n, "box": w["box"],
+                   "a_init": w["a_init"], "da": (w["a_end"] - w["a_init"]) / w["steps"],
+                   "parallelism": "single GPU"},
+        "fp64": {"value": v64, "ms_per_step": dt64 / fp64_steps * 1e3, "steps": fp64_steps},
+        "fp32_over_fp64": v32 / v64,
+        "pk_max_rel_err_fp32_vs_fp64_k_lt_half_nyquist": pk_err,
+        "step_roofline": {"bytes_per_step": step_bytes(n, n_p, 32),
+                          "achieved_GBs": step_bytes(n, n_p, 32) / (ms32 * 1e-3) / 1e9,
+                          "frac": step_bytes(n, n_p, 32) / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "stages_GBs": {k: round(sb[k] / (stage_ms[k] * 1e-3) / 1e9, 1) for k in stage_ms},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles(dom),
+                     "algorithmic_bytes_per_launch": sb[dom],
+                     "avg_launch_ms": stage_ms[dom], "launches": stage_calls[dom]},
+        "vs_reference_readme_cpu": v32 / 4.66e6,
+    }
+    if not args.no_cpu:
+        log("CPU baseline (port) on host cores")
+        out["cpu_baseline"] = cpu_baseline(pos, vel, w, args.cpu_steps, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
