@@ -46,13 +46,14 @@ def stage_bytes(n, n_p, prec):
     cells = n ** 3
     half = n * n * (n // 2 + 1) * 2 * r
     return {
-        "deposit": 3 * r * n_p + r * cells,            # read displacements, write rho
+        "bin": 3 * r * n_p + 4 * n_p,                   # read displacements, write >= 1 entry each
+        "deposit": 3 * r * n_p + 4 * n_p + r * cells,   # read entries + displacements, write rho
         "r2c_x": r * cells + half,                      # read rho, write half spectrum
         "c2c_y_fwd": 2 * half,
         "z_green": 2 * half,
         "c2c_y_inv": 2 * half,
         "c2r_x": half + r * cells,                      # read half spectrum, write phi
-        "gather_kick_drift": 12 * r * n_p + r * cells,  # read/write x, v; read phi
+        "gather_kick_drift": 12 * r * n_p + 4 * n_p + r * cells,  # entries, x, v in/out, phi
     }
 
 
@@ -63,12 +64,14 @@ def step_bytes(n, n_p, prec):
 
 
 def make_ics(w):
+    import contextlib
     import initial_conditions as ic
     p = dict(n_parts=w["n_parts"], n_cells=w["n_cells"], box=w["box"], seed=w["seed"], om=w["om"],
              ob=w["ob"], ol=w["ol"], ok=w["ok"], h=w["h"], ns=w["ns"], sigma8=w["sigma8"],
              a_init=w["a_init"])
-    delta = ic.gaussian_random_field(p)
-    return ic.zeldovich_numpy(delta, p)
+    with contextlib.redirect_stdout(sys.stderr):   # stdout carries only the JSON line
+        delta = ic.gaussian_random_field(p)
+        return ic.zeldovich_numpy(delta, p)
 
 
 def run_gpu(prec, pos, vel, w, steps, warmup):

@@ -368,8 +368,10 @@ struct pm_sim {
   void* Q = nullptr;
   const float* fgrid = nullptr;  // optional caller-owned Green's function grid
   FftTables* tab = nullptr;
+  TileGeom G{};
+  Bins B{};
   hipStream_t stream = nullptr;
-  bool has_state = false, rho_valid = false, phi_valid = false;
+  bool has_state = false, rho_valid = false, phi_valid = false, bins_valid = false;
   bool timing = false;
   struct Pending {
     int stage;
@@ -382,9 +384,16 @@ struct pm_sim {
 
 namespace {
 
+size_t bins_bytes(int n, int np_side, int prec) {
+  const TileGeom g = make_tiles(n, prec);
+  const size_t np = (size_t)np_side * np_side * np_side;
+  return 4 * (3 * 2 * (size_t)g.ntiles + 1 + (size_t)scan_blocks(g)) + 4 * 8 * np;
+}
+
 size_t sim_bytes(int n, int np_side, int prec) {
   const size_t nn = (size_t)n, np = (size_t)np_side * np_side * np_side;
-  return 6 * np * prec + nn * nn * nn * prec /*rho*/ + nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/;
+  return 6 * np * prec + nn * nn * nn * prec /*rho*/ + nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/ +
+         bins_bytes(n, np_side, prec);
 }
 
 struct StageTimer {
@@ -421,16 +430,29 @@ int sim_collect_timing(pm_sim* s) {
   return PM_OK;
 }
 
+// bin every particle into its tile entries from the current state
+template <typename R> int sim_bin_impl(pm_sim* s) {
+  StageTimer t(s, ST_BIN);
+  PM_HIP(hipMemsetAsync(s->B.counts, 0, 2 * sizeof(uint32_t) * s->G.ntiles, s->stream));
+  PM_HIP(bins_count<R>((const R*)s->s, s->L, s->G, s->B, s->stream));
+  PM_HIP(bins_scan(s->G, s->B, s->stream));
+  PM_HIP(bins_scatter<R>((const R*)s->s, s->L, s->G, s->B, s->stream));
+  s->bins_valid = true;
+  return PM_OK;
+}
+
 template <typename R> int sim_deposit_impl(pm_sim* s) {
+  if (!s->bins_valid)
+    if (int rc = sim_bin_impl<R>(s)) return rc;
   StageTimer t(s, ST_DEPOSIT);
-  PM_HIP(sim_deposit<R>((const R*)s->s, (R*)s->rho, s->mass, s->L, s->stream));
+  PM_HIP(deposit_tiles<R>((const R*)s->s, (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream));
   s->rho_valid = true;
   return PM_OK;
 }
 
 template <typename R> int sim_step_impl(pm_sim* s, double a, double da) {
   const int n = s->L.n;
-  if (!s->rho_valid)
+  if (!s->rho_valid || !s->bins_valid)
     if (int rc = sim_deposit_impl<R>(s)) return rc;
   cpx<R>* H = (cpx<R>*)s->H;
   cpx<R>* Q = (cpx<R>*)s->Q;
@@ -469,9 +491,15 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da) {
   const double drift = da / ((a + da) * (a + da)) * fa1;  // integrate.py:95
   {
     StageTimer t(s, ST_GKD);
-    PM_HIP(sim_gather_kick_drift<R>((R*)s->s, (R*)s->v, (const R*)s->H, kick, drift, s->L, s->stream));
+    PM_HIP(gkd_tiles<R>((R*)s->s, (R*)s->v, (const R*)s->H, kick, drift, s->L, s->G, s->B, s->stream));
   }
   s->rho_valid = false;
+  {
+    // the gather counted the entries of the new positions: finish their binning
+    StageTimer t(s, ST_BIN);
+    PM_HIP(bins_scan(s->G, s->B, s->stream));
+    PM_HIP(bins_scatter<R>((const R*)s->s, s->L, s->G, s->B, s->stream));
+  }
   return sim_deposit_impl<R>(s);
 }
 
@@ -492,6 +520,7 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
   *out = nullptr;
   if (int rc = check_grid(n_cells)) return rc;
   if (n_parts_side <= 0) return fail(PM_E_INVALID, "N_PARTS must be positive");
+  if (n_parts_side > 812) return fail(PM_E_INVALID, "N_PARTS above 812 is not supported (slot index range)");
   if (precision != 32 && precision != 64) return fail(PM_E_INVALID, "precision must be 32 or 64");
   pm_sim* s = new pm_sim();
   s->prec = precision / 8;
@@ -499,9 +528,13 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
   s->L.n = n_cells;
   s->L.np_side = n_parts_side;
   s->L.blk = (n_parts_side % 4 == 0) ? 4 : 1;
+  s->L.lb = s->L.blk == 4 ? 2 : 0;
   s->L.nb = n_parts_side / s->L.blk;
+  s->L.nbd = FastDiv((uint32_t)s->L.nb);
   s->L.np = (long)n_parts_side * n_parts_side * n_parts_side;
   s->L.r = (double)n_cells / (double)n_parts_side;
+  s->L.rf = (float)s->L.r;
+  s->L.r_int = (n_cells % n_parts_side) == 0 ? 1 : 0;
   s->mass = std::pow((double)n_cells / (double)n_parts_side, 3);  // pmesh.py:28
   const size_t nn = (size_t)n_cells, pr = (size_t)s->prec;
   auto cleanup = [&](int rc) {
@@ -514,6 +547,19 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
       hipMalloc(&s->rho, nn * nn * nn * pr) != hipSuccess || hipMalloc(&s->H, nn * nn * nn * pr) != hipSuccess ||
       hipMalloc(&s->Q, nn * nn * 2 * pr) != hipSuccess)
     return cleanup(fail(PM_E_NOMEM, "device allocation of %zu bytes failed", sim_bytes(n_cells, n_parts_side, s->prec)));
+  s->G = make_tiles(n_cells, s->prec);
+  const size_t m = 2 * (size_t)s->G.ntiles;
+  if (8 * (size_t)s->L.np >= (size_t)UINT32_MAX) return cleanup(fail(PM_E_INVALID, "too many particles (N_PARTS > 812)"));
+  s->B.capacity = (uint32_t)(8 * (size_t)s->L.np);
+  if (hipMalloc(&s->B.counts, m * 4) != hipSuccess || hipMalloc(&s->B.offsets, (m + 1) * 4) != hipSuccess ||
+      hipMalloc(&s->B.cursors, m * 4) != hipSuccess || hipMalloc(&s->B.entries, 8 * (size_t)s->L.np * 4) != hipSuccess ||
+      hipMalloc(&s->B.block_sums, (size_t)scan_blocks(s->G) * 4) != hipSuccess)
+    return cleanup(fail(PM_E_NOMEM, "device allocation of the tile bins failed"));
+  // every entry holds a valid slot from the start, counters start at zero
+  if (hipMemset(s->B.entries, 0, 8 * (size_t)s->L.np * 4) != hipSuccess ||
+      hipMemset(s->B.counts, 0, m * 4) != hipSuccess || hipMemset(s->B.cursors, 0, m * 4) != hipSuccess ||
+      hipMemset(s->B.offsets, 0, (m + 1) * 4) != hipSuccess)
+    return cleanup(fail(PM_E_HIP, "clearing the tile bins failed"));
   if (hipStreamCreate(&s->stream) != hipSuccess) return cleanup(fail(PM_E_HIP, "hipStreamCreate failed"));
   *out = s;
   return PM_OK;
@@ -531,6 +577,11 @@ PM_API int pm_sim_destroy(pm_sim* s) {
   (void)hipFree(s->rho);
   (void)hipFree(s->H);
   (void)hipFree(s->Q);
+  (void)hipFree(s->B.counts);
+  (void)hipFree(s->B.offsets);
+  (void)hipFree(s->B.cursors);
+  (void)hipFree(s->B.entries);
+  (void)hipFree(s->B.block_sums);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return PM_OK;
@@ -622,6 +673,7 @@ PM_API int pm_sim_set_state(pm_sim* s, const void* pos, const void* vel, int dty
   if (tmp) (void)hipFree(tmp);
   if (e != hipSuccess || es != hipSuccess) return fail(PM_E_HIP, "set_state failed: %s", hipGetErrorString(e ? e : es));
   s->has_state = true;
+  s->bins_valid = false;
   s->rho_valid = false;
   s->phi_valid = false;
   return PM_OK;

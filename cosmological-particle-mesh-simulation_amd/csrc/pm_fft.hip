@@ -51,11 +51,25 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const Tin* __restrict__ rho,
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
   const size_t row0 = (size_t)blockIdx.x * RB;
   const cpx<Tin>* src = reinterpret_cast<const cpx<Tin>*>(rho) + row0 * M;
-#pragma unroll 4
-  for (int e = tid; e < RB * M; e += NT) {
-    const int w = e / M, n = e - w * M;
-    const cpx<Tin> u = src[e];
-    lds[n * SN + w] = cmake<T>(R(u.x), R(u.y));
+  // the block's loads are issued in chunks of CH per thread before they are stored
+  constexpr int LPT = (RB * M + NT - 1) / NT;
+  constexpr int CH = sizeof(R) == 4 ? 32 : 8;
+#pragma unroll
+  for (int i0 = 0; i0 < LPT; i0 += CH) {
+    cpx<Tin> u[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int e = (i0 + i) * NT + tid;
+      if (i0 + i < LPT && e < RB * M) u[i] = src[e];
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int e = (i0 + i) * NT + tid;
+      if (i0 + i < LPT && e < RB * M) {
+        const int w = e / M, n = e - w * M;
+        lds[n * SN + w] = cmake<T>(R(u[i].x), R(u[i].y));
+      }
+    }
   }
   __syncthreads();
   fft_passes<T, M, 1, -1, RB, SN, NT, 2>(lds, tw, tid);
@@ -119,12 +133,26 @@ __global__ __launch_bounds__(kThreads) void k_c2r_x(const cpx<R>* H, const cpx<R
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
   const size_t row0 = (size_t)blockIdx.x * RB;
   const T* src = H + row0 * M;
-#pragma unroll 4
-  for (int e = tid; e < RB * M; e += NT) {
-    const int w = e / M, n = e - w * M;
-    lds[n * SN + w] = src[e];
-  }
+  constexpr int LPT = (RB * M + NT - 1) / NT;
+  constexpr int CH = sizeof(R) == 4 ? 32 : 8;
   if (tid < RB) qx[tid] = Q[row0 + tid];
+#pragma unroll
+  for (int i0 = 0; i0 < LPT; i0 += CH) {
+    T u[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int e = (i0 + i) * NT + tid;
+      if (i0 + i < LPT && e < RB * M) u[i] = src[e];
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int e = (i0 + i) * NT + tid;
+      if (i0 + i < LPT && e < RB * M) {
+        const int w = e / M, n = e - w * M;
+        lds[n * SN + w] = u[i];
+      }
+    }
+  }
   __syncthreads();
 
   constexpr int NPAIR = M / 2 + 1;
@@ -235,18 +263,42 @@ __global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, Penci
   } else {
     base = data + blk * (long)W * s.stride_w;
   }
-  // load
+  // load: the block's loads are issued in chunks of CH per thread before they are stored
+  constexpr int LPT = (L * W + NT - 1) / NT;
+  constexpr int CH = sizeof(R) == 4 ? 32 : 8;
   if (s.contig) {
-#pragma unroll 4
-    for (int e = tid; e < L * W; e += NT) {
-      const int n = e / W, w = e - n * W;
-      lds[n * SN + w] = base[n * s.stride_n + w];
+#pragma unroll
+    for (int i0 = 0; i0 < LPT; i0 += CH) {
+      T u[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const int e = (i0 + i) * NT + tid;
+        const int n = e / W, w = e - n * W;
+        if (i0 + i < LPT && e < L * W) u[i] = base[n * s.stride_n + w];
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const int e = (i0 + i) * NT + tid;
+        const int n = e / W, w = e - n * W;
+        if (i0 + i < LPT && e < L * W) lds[n * SN + w] = u[i];
+      }
     }
   } else {
-#pragma unroll 4
-    for (int e = tid; e < L * W; e += NT) {
-      const int w = e / L, n = e - w * L;
-      lds[n * SN + w] = base[n + w * s.stride_w];
+#pragma unroll
+    for (int i0 = 0; i0 < LPT; i0 += CH) {
+      T u[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const int e = (i0 + i) * NT + tid;
+        const int w = e / L, n = e - w * L;
+        if (i0 + i < LPT && e < L * W) u[i] = base[n + w * s.stride_w];
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const int e = (i0 + i) * NT + tid;
+        const int w = e / L, n = e - w * L;
+        if (i0 + i < LPT && e < L * W) lds[n * SN + w] = u[i];
+      }
     }
   }
   __syncthreads();

@@ -5,6 +5,26 @@
 
 namespace pm {
 
+// Division of 32-bit unsigned n < 2^31 by a fixed d >= 1 with one multiply-high and a shift
+// (round-up reciprocal, Granlund & Montgomery 1994).
+struct FastDiv {
+  uint32_t d = 1, m = 0, s = 0;
+  FastDiv() = default;
+  explicit FastDiv(uint32_t div) : d(div) {
+    s = 0;
+    while ((1ull << s) < div) ++s;
+    m = (uint32_t)((((1ull << 32) * ((1ull << s) - div)) / div) + 1);
+  }
+  __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    const uint32_t hi = __umulhi(n, m);
+#else
+    const uint32_t hi = (uint32_t)(((uint64_t)n * m) >> 32);
+#endif
+    return (uint32_t)(((uint64_t)hi + n) >> s);
+  }
+};
+
 // Device storage order of the particles of one simulation.
 //
 // The reference orders particles by their Lagrangian lattice site, p = (i*NP + j)*NP + k with
@@ -17,38 +37,81 @@ struct Lattice {
   int n;        // grid cells per side (N_CELLS)
   int np_side;  // particles per side (N_PARTS)
   int blk;      // block edge B (4, or 1 when N_PARTS % 4 != 0)
+  int lb;       // log2(B)
   int nb;       // blocks per side
+  FastDiv nbd;  // division by nb
   long np;      // N_PARTS^3
   double r;     // lattice spacing N_CELLS / N_PARTS
+  float rf;     // the same spacing in float32
+  int r_int;    // 1 when the spacing is an integer (then q is exact in float32 arithmetic)
 };
 
+// Lattice coordinate q = idx * N_CELLS/N_PARTS + 0.5 (zeldovich.py:79-81) in precision R.  The
+// float32 value is the float32 rounding of the exact one in both branches.
+template <typename R> __host__ __device__ __forceinline__ R lattice_coord(int idx, const Lattice& L) {
+  if (sizeof(R) == 4 && L.r_int) return R(fmaf((float)idx, L.rf, 0.5f));
+  return R((double)idx * L.r + 0.5);
+}
+
 __host__ __device__ __forceinline__ void slot_to_ijk(long slot, const Lattice& L, int& i, int& j, int& k) {
-  const int B = L.blk;
-  const int B3 = B * B * B;
-  const long blk = slot / B3;
-  const int loc = (int)(slot - blk * B3);
-  const int li = loc % B, lj = (loc / B) % B, lk = loc / (B * B);
-  const long bi = blk % L.nb;
-  const long t = blk / L.nb;
-  const long bj = t % L.nb, bk = t / L.nb;
-  i = (int)(bi * B + li);
-  j = (int)(bj * B + lj);
-  k = (int)(bk * B + lk);
+  const int lb = L.lb;
+  const uint32_t sl = (uint32_t)slot;
+  const uint32_t blk = sl >> (3 * lb);
+  const uint32_t loc = sl & ((1u << (3 * lb)) - 1u);
+  const uint32_t msk = (1u << lb) - 1u;
+  const uint32_t li = loc & msk, lj = (loc >> lb) & msk, lk = loc >> (2 * lb);
+  const uint32_t t = L.nbd.div(blk);
+  const uint32_t bi = blk - t * (uint32_t)L.nb;
+  const uint32_t bk = L.nbd.div(t);
+  const uint32_t bj = t - bk * (uint32_t)L.nb;
+  i = (int)((bi << lb) + li);
+  j = (int)((bj << lb) + lj);
+  k = (int)((bk << lb) + lk);
 }
 
 // Stage timers recorded by the sim (indices into the timing arrays).
-enum Stage { ST_DEPOSIT = 0, ST_R2C, ST_Y_FWD, ST_Z_GREEN, ST_Y_INV, ST_C2R, ST_GKD, ST_COUNT };
+enum Stage { ST_DEPOSIT = 0, ST_R2C, ST_Y_FWD, ST_Z_GREEN, ST_Y_INV, ST_C2R, ST_GKD, ST_BIN, ST_COUNT };
+
+// Eulerian tiles of the grid (cells per tile and tiles per axis).
+struct TileGeom {
+  int tx, ty, tz;
+  int ntx, nty, ntz;
+  int ntiles;
+};
+
+// Slot entries of every tile: own segment [offsets[2t], offsets[2t+1]) and halo segment
+// [offsets[2t+1], offsets[2t+2]); at most 8 entries per particle.
+struct Bins {
+  uint32_t* counts = nullptr;      // 2*ntiles, zero between steps
+  uint32_t* offsets = nullptr;     // 2*ntiles + 1
+  uint32_t* cursors = nullptr;     // 2*ntiles
+  uint32_t* entries = nullptr;     // 8*N_p
+  uint32_t* block_sums = nullptr;  // scan spine
+  uint32_t capacity = 0;           // entries allocated (8*N_p)
+};
+
+TileGeom make_tiles(int n, int prec_bytes);
+int scan_blocks(const TileGeom& g);
 
 // ---- resident-state kernels (slot order, displacement storage) -------------------------------
 template <typename R, typename Tin>
 hipError_t sim_set_state(const Tin* pos, const Tin* vel, R* s, R* v, const Lattice& L, hipStream_t st);
 template <typename R, typename Tout>
 hipError_t sim_get_state(const R* s, const R* v, Tout* pos, Tout* vel, const Lattice& L, hipStream_t st);
+
+// ---- tile binning, owner-computes deposit, tile-staged gather + kick + drift ------------------
 template <typename R>
-hipError_t sim_deposit(const R* s, R* rho, double mass, const Lattice& L, hipStream_t st);
+hipError_t bins_count(const R* s, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
+hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st);
 template <typename R>
-hipError_t sim_gather_kick_drift(R* s, R* v, const R* phi, double kick, double drift, const Lattice& L,
-                                 hipStream_t st);
+hipError_t bins_scatter(const R* s, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
+template <typename R>
+hipError_t deposit_tiles(const R* s, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
+                         hipStream_t st);
+// also counts the entries of the new positions into B.counts
+template <typename R>
+hipError_t gkd_tiles(R* s, R* v, const R* phi, double kick, double drift, const Lattice& L, const TileGeom& G,
+                     const Bins& B, hipStream_t st);
 
 // ---- reference-layout kernels (absolute float32 positions, reference order) ------------------
 hipError_t ref_density(const float* pos, long np, int n, double mass, float* rho, hipStream_t st);

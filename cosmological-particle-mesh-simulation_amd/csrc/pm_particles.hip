@@ -12,10 +12,7 @@ inline unsigned grid_for(long n, int block = kBlock) {
   return (unsigned)(g < 1 ? 1 : g);
 }
 
-template <typename R> __device__ __forceinline__ R lattice_q(int idx, double r) {
-  // zeldovich.py:79-81: linspace(0, N - N/NP, NP) + 0.5 has spacing N/NP
-  return R((double)idx * r + 0.5);
-}
+
 
 // ------------------------------------------------------------------------------------------
 // resident state <-> reference layout
@@ -50,106 +47,11 @@ __global__ __launch_bounds__(kBlock) void k_get_state(const R* __restrict__ s, c
   const long p = ((long)ijk[0] * L.np_side + ijk[1]) * L.np_side + ijk[2];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const R x = canonical_pos<R>(lattice_q<R>(ijk[d], L.r), s[d * L.np + slot], R(L.n));
+    const R x = canonical_pos<R>(lattice_coord<R>(ijk[d], L), s[d * L.np + slot], R(L.n));
     Tout xo = Tout(x);
     if (xo >= Tout(L.n)) xo = Tout(0);
     pos[d * L.np + p] = xo;
     vel[d * L.np + p] = Tout(v[d * L.np + slot]);
-  }
-}
-
-// canonical position of a resident particle, its cell and CIC fractions
-template <typename R>
-__device__ __forceinline__ void particle_cell(const R* __restrict__ s, long slot, const Lattice& L, int c[3],
-                                              R d[3]) {
-  int ijk[3];
-  slot_to_ijk(slot, L, ijk[0], ijk[1], ijk[2]);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const R x = canonical_pos<R>(lattice_q<R>(ijk[a], L.r), s[a * L.np + slot], R(L.n));
-    const int ci = cell_of<R>(x, L.n);
-    c[a] = ci;
-    d[a] = x - R(ci);
-  }
-}
-
-// ------------------------------------------------------------------------------------------
-// CIC deposit from the resident state (density.py:17-47), race-free through atomics
-// ------------------------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_deposit_atomic(const R* __restrict__ s, R* __restrict__ rho, R mass,
-                                                           Lattice L) {
-  const long slot = blockIdx.x * (long)kBlock + threadIdx.x;
-  if (slot >= L.np) return;
-  int c[3];
-  R d[3];
-  particle_cell<R>(s, slot, L, c, d);
-  const int n = L.n;
-  const int X = c[0] + 1 == n ? 0 : c[0] + 1;
-  const int Y = c[1] + 1 == n ? 0 : c[1] + 1;
-  const int Z = c[2] + 1 == n ? 0 : c[2] + 1;
-  const R tx = R(1) - d[0], ty = R(1) - d[1], tz = R(1) - d[2];
-  const size_t nn = (size_t)n;
-  auto at = [&](int z, int y, int x) { return rho + ((size_t)z * nn + y) * nn + x; };
-  atomicAdd(at(c[2], c[1], c[0]), mass * tx * ty * tz);
-  atomicAdd(at(c[2], c[1], X), mass * d[0] * ty * tz);
-  atomicAdd(at(c[2], Y, c[0]), mass * tx * d[1] * tz);
-  atomicAdd(at(Z, c[1], c[0]), mass * tx * ty * d[2]);
-  atomicAdd(at(c[2], Y, X), mass * d[0] * d[1] * tz);
-  atomicAdd(at(Z, Y, c[0]), mass * tx * d[1] * d[2]);
-  atomicAdd(at(Z, c[1], X), mass * d[0] * ty * d[2]);
-  atomicAdd(at(Z, Y, X), mass * d[0] * d[1] * d[2]);
-}
-
-// ------------------------------------------------------------------------------------------
-// fused CIC gather + kick + drift (integrate.py:15-97)
-//
-// For axis a the reference gathers g_a = 1/2 sum_corners t_j [phi(c_j - e_a) - phi(c_j + e_a)]
-// (integrate.py:61-92) at pre-step positions for all three axes (integrate.py:20-24), kicks
-// v_a += da*f(a+da)*g_a (:94) and drifts x_a += da*v_a/(a+da)^2*f(a+da) (:95).  Here the 32
-// distinct phi samples of the 4x4x4 stencil are loaded once and all three axes are updated
-// in one pass; the displacement is kept in [-N/2, N/2).
-// ------------------------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_gkd(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi,
-                                                R kick, R drift, Lattice L) {
-  const long slot = blockIdx.x * (long)kBlock + threadIdx.x;
-  if (slot >= L.np) return;
-  int c[3];
-  R d[3];
-  particle_cell<R>(s, slot, L, c, d);
-  const int n = L.n;
-  int xi[4], yi[4], zi[4];
-#pragma unroll
-  for (int o = 0; o < 4; ++o) {
-    xi[o] = wrapi(c[0] + o - 1, n);
-    yi[o] = wrapi(c[1] + o - 1, n);
-    zi[o] = wrapi(c[2] + o - 1, n);
-  }
-  const size_t nn = (size_t)n;
-  auto P = [&](int zz, int yy, int xx) { return phi[((size_t)zi[zz] * nn + yi[yy]) * nn + xi[xx]]; };
-  const R w0[2] = {R(1) - d[0], d[0]};
-  const R w1[2] = {R(1) - d[1], d[1]};
-  const R w2[2] = {R(1) - d[2], d[2]};
-  R g[3] = {R(0), R(0), R(0)};
-#pragma unroll
-  for (int oz = 0; oz < 2; ++oz)
-#pragma unroll
-    for (int oy = 0; oy < 2; ++oy)
-#pragma unroll
-      for (int ox = 0; ox < 2; ++ox) {
-        const R w = w0[ox] * w1[oy] * w2[oz];
-        g[0] += w * (P(oz + 1, oy + 1, ox) - P(oz + 1, oy + 1, ox + 2));
-        g[1] += w * (P(oz + 1, oy, ox + 1) - P(oz + 1, oy + 2, ox + 1));
-        g[2] += w * (P(oz, oy + 1, ox + 1) - P(oz + 2, oy + 1, ox + 1));
-      }
-  const R nr = R(n);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    R va = v[a * L.np + slot] + kick * (R(0.5) * g[a]);
-    R sa = s[a * L.np + slot] + drift * va;
-    v[a * L.np + slot] = va;
-    s[a * L.np + slot] = wrap_disp<R>(sa, nr);
   }
 }
 
@@ -380,21 +282,6 @@ hipError_t sim_get_state(const R* s, const R* v, Tout* pos, Tout* vel, const Lat
   k_get_state<R, Tout><<<grid_for(L.np), kBlock, 0, st>>>(s, v, pos, vel, L);
   return hipGetLastError();
 }
-template <typename R>
-hipError_t sim_deposit(const R* s, R* rho, double mass, const Lattice& L, hipStream_t st) {
-  const size_t bytes = (size_t)L.n * L.n * L.n * sizeof(R);
-  hipError_t e = hipMemsetAsync(rho, 0, bytes, st);
-  if (e != hipSuccess) return e;
-  k_deposit_atomic<R><<<grid_for(L.np), kBlock, 0, st>>>(s, rho, R(mass), L);
-  return hipGetLastError();
-}
-template <typename R>
-hipError_t sim_gather_kick_drift(R* s, R* v, const R* phi, double kick, double drift, const Lattice& L,
-                                 hipStream_t st) {
-  k_gkd<R><<<grid_for(L.np), kBlock, 0, st>>>(s, v, phi, R(kick), R(drift), L);
-  return hipGetLastError();
-}
-
 template hipError_t sim_set_state<float, float>(const float*, const float*, float*, float*, const Lattice&, hipStream_t);
 template hipError_t sim_set_state<float, double>(const double*, const double*, float*, float*, const Lattice&,
                                                  hipStream_t);
@@ -408,12 +295,6 @@ template hipError_t sim_get_state<float, double>(const float*, const float*, dou
 template hipError_t sim_get_state<double, float>(const double*, const double*, float*, float*, const Lattice&,
                                                  hipStream_t);
 template hipError_t sim_get_state<double, double>(const double*, const double*, double*, double*, const Lattice&,
-                                                  hipStream_t);
-template hipError_t sim_deposit<float>(const float*, float*, double, const Lattice&, hipStream_t);
-template hipError_t sim_deposit<double>(const double*, double*, double, const Lattice&, hipStream_t);
-template hipError_t sim_gather_kick_drift<float>(float*, float*, const float*, double, double, const Lattice&,
-                                                 hipStream_t);
-template hipError_t sim_gather_kick_drift<double>(double*, double*, const double*, double, double, const Lattice&,
                                                   hipStream_t);
 
 hipError_t ref_density(const float* pos, long np, int n, double mass, float* rho, hipStream_t st) {

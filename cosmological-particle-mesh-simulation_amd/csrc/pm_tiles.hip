@@ -1,0 +1,592 @@
+// Eulerian tiles for the PM step: per-step counting sort of particle slots into grid tiles,
+// owner-computes CIC deposit and the tile-staged gather + kick + drift (gfx950).
+//
+// Each tile of TX x TY x TZ cells owns two segments of slot entries:
+//   own  : particles whose cell floor(x) lies in the tile;
+//   halo : particles of the lower neighbour tiles whose +1 CIC corner falls in this tile
+//          (a particle on the upper face of its tile is entered once more for every tile its
+//          cloud reaches, at most 8 entries in all).
+// The deposit of a tile reads own + halo entries, accumulates the in-tile corners in LDS and
+// writes every cell of the tile with plain stores: no global float atomics, no zero fill.
+// The LDS accumulator is FP64 in both precisions: on gfx950 ds_add_f64 sustains about twice
+// the rate of ds_add_f32 for this access pattern (measured 0.33 ms against 0.76 ms per
+// 256^3-particle deposit), and the sums are then rounded once instead of at every add.
+// The gather reads the own entries with the tile's potential (plus a -1/+2 stencil margin)
+// staged in LDS.  Counts come from a wave-aggregated atomic per distinct key (one atomic per
+// distinct tile in a wavefront), and the offsets from a three-kernel exclusive scan.
+#include "pm_particles.h"
+
+namespace pm {
+
+namespace {
+
+constexpr int kT = 256;  // threads per block
+
+inline unsigned blocks_for(long n, int b = kT) {
+  const long g = (n + b - 1) / b;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+
+// index in [-n, 2n) folded into [0, n)
+__device__ __forceinline__ int fold(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// canonical position of a resident particle from its displacement: cell, CIC fractions and
+// lattice point q per axis
+template <typename R>
+__device__ __forceinline__ void cell_and_frac(R sx, R sy, R sz, long slot, const Lattice& L, int c[3], R d[3],
+                                              R q[3]) {
+  int ijk[3];
+  slot_to_ijk(slot, L, ijk[0], ijk[1], ijk[2]);
+  const R sv[3] = {sx, sy, sz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    q[a] = lattice_coord<R>(ijk[a], L);
+    const R x = canonical_pos<R>(q[a], sv[a], R(L.n));
+    c[a] = cell_of<R>(x, L.n);
+    d[a] = x - R(c[a]);
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void cell_and_frac(const R* __restrict__ s, long slot, const Lattice& L, int c[3], R d[3]) {
+  R q[3];
+  cell_and_frac<R>(s[slot], s[L.np + slot], s[2 * L.np + slot], slot, L, c, d, q);
+}
+
+// home tile of a cell and whether the cloud reaches the next tile along each axis
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ void tile_of(const int c[3], const TileGeom& G, int t[3], int up[3]) {
+  t[0] = c[0] / TX;
+  t[1] = c[1] / TY;
+  t[2] = c[2] / TZ;
+  // with a single tile along an axis the wrapped corner stays in the home tile
+  up[0] = (c[0] % TX == TX - 1) && G.ntx > 1;
+  up[1] = (c[1] % TY == TY - 1) && G.nty > 1;
+  up[2] = (c[2] % TZ == TZ - 1) && G.ntz > 1;
+}
+
+// key of corner-offset o (bit 0: x, 1: y, 2: z) of a particle in tile t: 2*tile (+1 for halo)
+__device__ __forceinline__ uint32_t target_key(const int t[3], int o, const TileGeom& G) {
+  int x = t[0] + (o & 1), y = t[1] + ((o >> 1) & 1), z = t[2] + (o >> 2);
+  if (x == G.ntx) x = 0;
+  if (y == G.nty) y = 0;
+  if (z == G.ntz) z = 0;
+  const uint32_t tile = (uint32_t)((z * G.nty + y) * G.ntx + x);
+  return 2u * tile + (o != 0 ? 1u : 0u);
+}
+
+__device__ __forceinline__ bool target_active(const int up[3], int o) {
+  return ((o & 1) <= up[0]) && (((o >> 1) & 1) <= up[1]) && ((o >> 2) <= up[2]);
+}
+
+// Adds 1 to ctr[key] for every active lane with one atomic per distinct key in the wavefront
+// and returns each lane's rank among all additions to its key (old value + lane prefix).
+// Must be reached by every lane of the wavefront.
+template <bool RESULT>
+__device__ __forceinline__ uint32_t wave_add(uint32_t* ctr, uint32_t key, bool active) {
+  const int lane = __lane_id();
+  unsigned long long pending = __ballot(active);
+  uint32_t res = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t lkey = __shfl(key, leader);
+    const unsigned long long same = __ballot(active && key == lkey) & pending;
+    if (RESULT) {
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(same));
+      base = __shfl(base, leader);
+      if ((same >> lane) & 1ull) res = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    } else {
+      if (lane == leader) atomicAdd(&ctr[lkey], (uint32_t)__popcll(same));
+    }
+    pending &= ~same;
+  }
+  return res;
+}
+
+// ------------------------------------------------------------------------------------------
+// counting and scattering slot entries
+// ------------------------------------------------------------------------------------------
+template <typename R, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kT) void k_bin_count(const R* __restrict__ s, Lattice L, TileGeom G,
+                                                  uint32_t* __restrict__ counts) {
+  const long slot = blockIdx.x * (long)kT + threadIdx.x;
+  const bool valid = slot < L.np;
+  int c[3] = {0, 0, 0}, t[3] = {0, 0, 0}, up[3] = {0, 0, 0};
+  R d[3];
+  if (valid) {
+    cell_and_frac<R>(s, slot, L, c, d);
+    tile_of<TX, TY, TZ>(c, G, t, up);
+  }
+#pragma unroll 1
+  for (int o = 0; o < 8; ++o) {
+    const bool act = valid && target_active(up, o);
+    wave_add<false>(counts, act ? target_key(t, o, G) : 0u, act);
+  }
+}
+
+template <typename R, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, Lattice L, TileGeom G,
+                                                    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursors,
+                                                    uint32_t* __restrict__ entries, uint32_t capacity) {
+  const long slot = blockIdx.x * (long)kT + threadIdx.x;
+  const bool valid = slot < L.np;
+  int c[3] = {0, 0, 0}, t[3] = {0, 0, 0}, up[3] = {0, 0, 0};
+  R d[3];
+  if (valid) {
+    cell_and_frac<R>(s, slot, L, c, d);
+    tile_of<TX, TY, TZ>(c, G, t, up);
+  }
+#pragma unroll 1
+  for (int o = 0; o < 8; ++o) {
+    const bool act = valid && target_active(up, o);
+    const uint32_t key = act ? target_key(t, o, G) : 0u;
+    const uint32_t rank = wave_add<true>(cursors, key, act);
+    if (act) {
+      const uint32_t pos = offsets[key] + rank;
+      if (pos < offsets[key + 1] && pos < capacity) entries[pos] = (uint32_t)slot;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// exclusive scan of the 2*ntiles counts -> offsets (+ total at the end); clears the counters
+// ------------------------------------------------------------------------------------------
+constexpr int kScanPer = 16;
+constexpr int kScanChunk = kT * kScanPer;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < kT; off <<= 1) {
+    const uint32_t t = tid >= off ? sh[tid - off] : 0u;
+    __syncthreads();
+    sh[tid] += t;
+    __syncthreads();
+  }
+  const uint32_t incl = sh[tid];
+  total = sh[kT - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kT) void k_scan_reduce(const uint32_t* __restrict__ counts, long m,
+                                                    uint32_t* __restrict__ bsum) {
+  __shared__ uint32_t sh[kT];
+  const long base = blockIdx.x * (long)kScanChunk;
+  uint32_t v = 0;
+  for (int i = 0; i < kScanPer; ++i) {
+    const long j = base + (long)i * kT + threadIdx.x;
+    if (j < m) v += counts[j];
+  }
+  uint32_t total;
+  block_exclusive_scan(v, sh, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kT) void k_scan_spine(uint32_t* __restrict__ bsum, int nb) {
+  __shared__ uint32_t sh[kT];
+  uint32_t carry = 0;
+  for (int base = 0; base < nb; base += kT) {
+    const int j = base + threadIdx.x;
+    const uint32_t v = j < nb ? bsum[j] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, sh, total);
+    if (j < nb) bsum[j] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_scan_apply(uint32_t* __restrict__ counts, long m,
+                                                   const uint32_t* __restrict__ bsum, uint32_t* __restrict__ offsets,
+                                                   uint32_t* __restrict__ cursors) {
+  __shared__ uint32_t sh[kT];
+  const long base = blockIdx.x * (long)kScanChunk + (long)threadIdx.x * kScanPer;
+  uint32_t v[kScanPer];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    const long j = base + i;
+    v[i] = j < m ? counts[j] : 0u;
+    sum += v[i];
+  }
+  uint32_t total;
+  uint32_t run = bsum[blockIdx.x] + block_exclusive_scan(sum, sh, total);
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    const long j = base + i;
+    if (j < m) {
+      offsets[j] = run;
+      counts[j] = 0u;
+      cursors[j] = 0u;
+    }
+    run += v[i];
+    if (j == m - 1) offsets[m] = run;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// owner-computes CIC deposit (density.py:17-47 for the cells of one tile)
+//
+// The entry and displacement loads of E particles per thread are issued together before any
+// is used, so each pass over a tile's entries costs two memory latencies, not 2E.
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t kNone = 0xffffffffu;
+
+template <typename R, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R* __restrict__ rho, R mass,
+                                                      Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
+                                                      const uint32_t* __restrict__ entries, uint32_t capacity) {
+  constexpr int CELLS = TX * TY * TZ;
+  static_assert(CELLS % kT == 0, "tile must hold a multiple of the block size");
+  constexpr int E = 4;
+  // row pitch TX + 3: consecutive lattice particles (2 cells apart) then spread over the LDS
+  // banks instead of sharing the few that a power-of-two pitch leaves them
+  constexpr int PX = TX + 3, PLANE = PX * TY, PADDED = PLANE * TZ;
+  __shared__ double acc[PADDED];
+  for (int i = threadIdx.x; i < PADDED; i += kT) acc[i] = 0.0;
+  const int tile = blockIdx.x;
+  const int tix = tile % G.ntx, tiy = (tile / G.ntx) % G.nty, tiz = tile / (G.ntx * G.nty);
+  const int o0[3] = {tix * TX, tiy * TY, tiz * TZ};
+  const int T[3] = {TX, TY, TZ};
+  const int n = L.n;
+  const long np = L.np;
+  const uint32_t end = min(offsets[2 * tile + 2], capacity);
+  const uint32_t beg = min(offsets[2 * tile], end);
+  __syncthreads();
+  for (uint32_t base = beg; base < end; base += kT * E) {
+    uint32_t sl[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const uint32_t e = base + u * kT + threadIdx.x;
+      sl[u] = e < end ? min(entries[e], (uint32_t)(np - 1)) : kNone;
+    }
+    R sv[E][3];
+#pragma unroll
+    for (int u = 0; u < E; ++u)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) sv[u][a] = sl[u] != kNone ? s[a * np + sl[u]] : R(0);
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      if (sl[u] == kNone) continue;
+      int c[3];
+      R d[3], q[3];
+      cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], sl[u], L, c, d, q);
+      int r0[3], r1[3];  // tile-relative coordinates of the two corners per axis (-1: outside)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        int lo = c[a] - o0[a];
+        if (lo < 0) lo += n;
+        int hi = (c[a] + 1 == n ? 0 : c[a] + 1) - o0[a];
+        if (hi < 0) hi += n;
+        r0[a] = lo < T[a] ? lo : -1;
+        r1[a] = hi < T[a] ? hi : -1;
+      }
+      // density.py:24-47: weights t = 1 - d and d, products with the mass in FP64
+      const double dd[3] = {(double)d[0], (double)d[1], (double)d[2]};
+      const double w0[3] = {1.0 - dd[0], 1.0 - dd[1], 1.0 - dd[2]};
+#pragma unroll
+      for (int oz = 0; oz < 2; ++oz) {
+        const int rz = oz ? r1[2] : r0[2];
+        if (rz < 0) continue;
+        const double wz = (double)mass * (oz ? dd[2] : w0[2]);
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) {
+          const int ry = oy ? r1[1] : r0[1];
+          if (ry < 0) continue;
+          const double wyz = wz * (oy ? dd[1] : w0[1]);
+#pragma unroll
+          for (int ox = 0; ox < 2; ++ox) {
+            const int rx = ox ? r1[0] : r0[0];
+            if (rx < 0) continue;
+            __hip_atomic_fetch_add(&acc[rz * PLANE + ry * PX + rx], wyz * (ox ? dd[0] : w0[0]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const size_t nn = (size_t)n;
+#pragma unroll
+  for (int i0 = 0; i0 < CELLS / kT; ++i0) {
+    const int i = i0 * kT + threadIdx.x;
+    const int rx = i % TX, ry = (i / TX) % TY, rz = i / (TX * TY);
+    rho[((size_t)(o0[2] + rz) * nn + (o0[1] + ry)) * nn + (o0[0] + rx)] = R(acc[rz * PLANE + ry * PX + rx]);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// tile-staged gather + kick + drift (integrate.py:15-97), counting the new tiles
+//
+// For axis a the reference gathers g_a = 1/2 sum_corners t_j [phi(c_j - e_a) - phi(c_j + e_a)]
+// (integrate.py:61-92) at pre-step positions for all three axes (integrate.py:20-24), kicks
+// v_a += da*f(a+da)*g_a (:94) and drifts x_a += da*v_a/(a+da)^2*f(a+da) (:95).  Here the 32
+// distinct phi samples of the 4x4x4 stencil come from the LDS copy of the tile, all three
+// axes are updated in one pass, the displacement is kept in [-N/2, N/2), and the entries of
+// the new position are counted for the next binning.  The potential box is fetched in
+// batches of CH loads per thread before they are written to LDS.
+// ------------------------------------------------------------------------------------------
+template <typename R, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi,
+                                                  R kick, R drift, Lattice L, TileGeom G,
+                                                  const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
+                                                  uint32_t capacity) {
+  constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, BN = BX * BY * BZ;
+  static_assert(BX <= 64, "a box row must fit in one wavefront");
+  constexpr int RPW = 64 / BX;                      // box rows per wavefront load
+  constexpr int NW = kT / 64;                       // wavefronts per block
+  constexpr int ROWS = BY * BZ;
+  constexpr int ITERS = (ROWS + RPW * NW - 1) / (RPW * NW);
+  constexpr int E = 2;
+  __shared__ R box[BN];
+  __shared__ uint32_t hist[128];  // new-tile counts: offset (-1..2)^3 from this tile x {own, halo}
+  const int tile = blockIdx.x;
+  const uint32_t end = min(offsets[2 * tile + 1], capacity);
+  const uint32_t beg = min(offsets[2 * tile], end);
+  if (beg == end) return;  // empty tile: the whole block leaves together
+  const int tix = tile % G.ntx, tiy = (tile / G.ntx) % G.nty, tiz = tile / (G.ntx * G.nty);
+  const int o0[3] = {tix * TX, tiy * TY, tiz * TZ};
+  const int n = L.n;
+  const long np = L.np;
+  const size_t nn = (size_t)n;
+  // the local histogram needs 4 distinct tiles per axis to name the neighbourhood uniquely
+  const bool local_hist = G.ntx >= 4 && G.nty >= 4 && G.ntz >= 4;
+  if (threadIdx.x < 128) hist[threadIdx.x] = 0u;
+  // potential of the tile and its -1/+2 stencil margin (periodic), one box row per lane group
+  {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rl = lane / BX, x = lane - rl * BX;
+    const bool lane_on = rl < RPW;
+    const int gx = fold(o0[0] - 1 + x, n);
+    R tmp[ITERS];
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+      const int row = RPW * (wv + NW * it) + rl;
+      if (lane_on && row < ROWS) {
+        const int by = row % BY, bz = row / BY;
+        const int gy = fold(o0[1] - 1 + by, n), gz = fold(o0[2] - 1 + bz, n);
+        tmp[it] = phi[((size_t)gz * nn + gy) * nn + gx];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+      const int row = RPW * (wv + NW * it) + rl;
+      if (lane_on && row < ROWS) box[row * BX + x] = tmp[it];
+    }
+  }
+  __syncthreads();
+  const R nr = R(n);
+  for (uint32_t base = beg; base < end; base += kT * E) {
+    uint32_t sl[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const uint32_t e = base + u * kT + threadIdx.x;
+      sl[u] = e < end ? min(entries[e], (uint32_t)(np - 1)) : kNone;
+    }
+    R sv[E][3], vv[E][3];
+#pragma unroll
+    for (int u = 0; u < E; ++u)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        sv[u][a] = sl[u] != kNone ? s[a * np + sl[u]] : R(0);
+        vv[u][a] = sl[u] != kNone ? v[a * np + sl[u]] : R(0);
+      }
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const bool valid = sl[u] != kNone;
+      int tn[3] = {0, 0, 0}, up[3] = {0, 0, 0};
+      if (valid) {
+        int c[3];
+        R d[3], q[3];
+        cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], sl[u], L, c, d, q);
+        int b[3];
+        bool inside = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          int rel = c[a] - o0[a];
+          if (rel < 0) rel += n;
+          b[a] = rel;  // box coordinate of the cell is rel + 1
+          inside = inside && rel < (a == 0 ? TX : (a == 1 ? TY : TZ));
+        }
+        R P[4][4][4];
+#pragma unroll
+        for (int zz = 0; zz < 4; ++zz)
+#pragma unroll
+          for (int yy = 0; yy < 4; ++yy)
+#pragma unroll
+            for (int xx = 0; xx < 4; ++xx) {
+              const int outside = (xx == 0 || xx == 3) + (yy == 0 || yy == 3) + (zz == 0 || zz == 3);
+              if (outside <= 1) {
+                if (inside)
+                  P[zz][yy][xx] = box[((b[2] + zz) * BY + (b[1] + yy)) * BX + (b[0] + xx)];
+                else  // a particle outside its own tile: inconsistent bins, read the grid itself
+                  P[zz][yy][xx] =
+                      phi[((size_t)wrapi(c[2] + zz - 1, n) * nn + wrapi(c[1] + yy - 1, n)) * nn + wrapi(c[0] + xx - 1, n)];
+              }
+            }
+        const R w0[2] = {R(1) - d[0], d[0]};
+        const R w1[2] = {R(1) - d[1], d[1]};
+        const R w2[2] = {R(1) - d[2], d[2]};
+        R g[3] = {R(0), R(0), R(0)};
+#pragma unroll
+        for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+          for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+            for (int ox = 0; ox < 2; ++ox) {
+              const R w = w0[ox] * w1[oy] * w2[oz];
+              g[0] += w * (P[oz + 1][oy + 1][ox] - P[oz + 1][oy + 1][ox + 2]);
+              g[1] += w * (P[oz + 1][oy][ox + 1] - P[oz + 1][oy + 2][ox + 1]);
+              g[2] += w * (P[oz][oy + 1][ox + 1] - P[oz + 2][oy + 1][ox + 1]);
+            }
+        int cn[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const R va = vv[u][a] + kick * (R(0.5) * g[a]);
+          const R sa = wrap_disp<R>(sv[u][a] + drift * va, nr);
+          v[a * np + sl[u]] = va;
+          s[a * np + sl[u]] = sa;
+          cn[a] = cell_of<R>(canonical_pos<R>(q[a], sa, nr), n);
+        }
+        tile_of<TX, TY, TZ>(cn, G, tn, up);
+        if (local_hist) {
+          const int tc[3] = {tix, tiy, tiz};
+          const int ntc[3] = {G.ntx, G.nty, G.ntz};
+#pragma unroll 1
+          for (int o = 0; o < 8; ++o) {
+            if (!target_active(up, o)) continue;
+            const int oo[3] = {o & 1, (o >> 1) & 1, o >> 2};
+            int rr[3];
+            bool near = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              int dlt = tn[a] + oo[a] - tc[a];  // periodic difference folded into [-1, 2]
+              if (dlt > 2) dlt -= ntc[a];
+              if (dlt < -1) dlt += ntc[a];
+              rr[a] = dlt + 1;
+              near = near && dlt >= -1 && dlt <= 2;
+            }
+            if (near) atomicAdd(&hist[2 * ((rr[2] * 4 + rr[1]) * 4 + rr[0]) + (o != 0)], 1u);
+            else atomicAdd(&counts[target_key(tn, o, G)], 1u);  // moved beyond the neighbourhood
+          }
+        }
+      }
+      if (!local_hist) {
+#pragma unroll 1
+        for (int o = 0; o < 8; ++o) {
+          const bool act = valid && target_active(up, o);
+          wave_add<false>(counts, act ? target_key(tn, o, G) : 0u, act);
+        }
+      }
+    }
+  }
+  if (local_hist) {
+    __syncthreads();
+    if (threadIdx.x < 128) {
+      const uint32_t cnt = hist[threadIdx.x];
+      if (cnt) {
+        const int cell = threadIdx.x >> 1, kind = threadIdx.x & 1;
+        const int rx = cell & 3, ry = (cell >> 2) & 3, rz = cell >> 4;
+        const int x = fold(tix + rx - 1, G.ntx), y = fold(tiy + ry - 1, G.nty), z = fold(tiz + rz - 1, G.ntz);
+        atomicAdd(&counts[2u * (uint32_t)((z * G.nty + y) * G.ntx + x) + kind], cnt);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+TileGeom make_tiles(int n, int prec_bytes) {
+  auto pick = [n](int pref) {
+    int t = pref;
+    while (t > 1 && n % t) t /= 2;
+    return t;
+  };
+  TileGeom g;
+  (void)prec_bytes;  // 16^3 tiles in both precisions (FP64 LDS accumulators in the deposit)
+  g.tx = pick(16);
+  g.ty = pick(16);
+  g.tz = pick(16);
+  if (g.ty < 8 || g.tz < 8 || g.tx < 8) g.tx = g.ty = g.tz = 0;  // not a supported shape
+  g.ntx = g.tx ? n / g.tx : 0;
+  g.nty = g.ty ? n / g.ty : 0;
+  g.ntz = g.tz ? n / g.tz : 0;
+  g.ntiles = g.ntx * g.nty * g.ntz;
+  return g;
+}
+
+int scan_blocks(const TileGeom& g) { return (int)((2L * g.ntiles + kScanChunk - 1) / kScanChunk); }
+
+// tile shapes the kernels are instantiated for; make_tiles only produces these
+#define PM_TILE_DISPATCH(G, CALL)                                        \
+  do {                                                                   \
+    if ((G).tx == 32 && (G).ty == 16 && (G).tz == 16) CALL(32, 16, 16);  \
+    else if ((G).tx == 16 && (G).ty == 16 && (G).tz == 16) CALL(16, 16, 16); \
+    else if ((G).tx == 8 && (G).ty == 8 && (G).tz == 8) CALL(8, 8, 8);   \
+    else if ((G).tx == 16 && (G).ty == 8 && (G).tz == 8) CALL(16, 8, 8); \
+    else return hipErrorInvalidValue;                                    \
+  } while (0)
+
+template <typename R>
+hipError_t bins_count(const R* s, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st) {
+#define PM_L(X, Y, Z) k_bin_count<R, X, Y, Z><<<blocks_for(L.np), kT, 0, st>>>(s, L, G, B.counts)
+  PM_TILE_DISPATCH(G, PM_L);
+#undef PM_L
+  return hipGetLastError();
+}
+
+hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st) {
+  const long m = 2L * G.ntiles;
+  const int nb = scan_blocks(G);
+  k_scan_reduce<<<nb, kT, 0, st>>>(B.counts, m, B.block_sums);
+  k_scan_spine<<<1, kT, 0, st>>>(B.block_sums, nb);
+  k_scan_apply<<<nb, kT, 0, st>>>(B.counts, m, B.block_sums, B.offsets, B.cursors);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t bins_scatter(const R* s, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st) {
+#define PM_L(X, Y, Z) \
+  k_bin_scatter<R, X, Y, Z><<<blocks_for(L.np), kT, 0, st>>>(s, L, G, B.offsets, B.cursors, B.entries, B.capacity)
+  PM_TILE_DISPATCH(G, PM_L);
+#undef PM_L
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t deposit_tiles(const R* s, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
+                         hipStream_t st) {
+#define PM_L(X, Y, Z) \
+  k_deposit_tiles<R, X, Y, Z><<<G.ntiles, kT, 0, st>>>(s, rho, R(mass), L, G, B.offsets, B.entries, B.capacity)
+  PM_TILE_DISPATCH(G, PM_L);
+#undef PM_L
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t gkd_tiles(R* s, R* v, const R* phi, double kick, double drift, const Lattice& L, const TileGeom& G,
+                     const Bins& B, hipStream_t st) {
+#define PM_L(X, Y, Z)                                                                                       \
+  k_gkd_tiles<R, X, Y, Z><<<G.ntiles, kT, 0, st>>>(s, v, phi, R(kick), R(drift), L, G, B.offsets, B.entries, \
+                                                   B.counts, B.capacity)
+  PM_TILE_DISPATCH(G, PM_L);
+#undef PM_L
+  return hipGetLastError();
+}
+
+#define PM_INST(R)                                                                                           \
+  template hipError_t bins_count<R>(const R*, const Lattice&, const TileGeom&, const Bins&, hipStream_t);    \
+  template hipError_t bins_scatter<R>(const R*, const Lattice&, const TileGeom&, const Bins&, hipStream_t);  \
+  template hipError_t deposit_tiles<R>(const R*, R*, double, const Lattice&, const TileGeom&, const Bins&,   \
+                                       hipStream_t);                                                         \
+  template hipError_t gkd_tiles<R>(R*, R*, const R*, double, double, const Lattice&, const TileGeom&,        \
+                                   const Bins&, hipStream_t);
+PM_INST(float)
+PM_INST(double)
+#undef PM_INST
+
+}  // namespace pm

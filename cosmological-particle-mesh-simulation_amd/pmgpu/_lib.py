@@ -17,7 +17,8 @@ PM_FLAG_CIC_DECONVOLVE = 1
 PM_FLAG_FIX_F_COSMOLOGY = 2
 
 # N stages timed by pm_sim_get_timing, in order
-STAGES = ("deposit", "r2c_x", "c2c_y_fwd", "z_green", "c2c_y_inv", "c2r_x", "gather_kick_drift")
+STAGES = ("deposit", "r2c_x", "c2c_y_fwd", "z_green", "c2c_y_inv", "c2r_x", "gather_kick_drift",
+          "bin")
 
 _vp = C.c_void_p
 _i = C.c_int

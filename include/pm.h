@@ -120,7 +120,7 @@ PM_API int pm_sim_step(pm_sim* sim, double a, double da);
 /* n_steps steps from a with step da, enqueued back to back; returns the final a.            */
 PM_API int pm_sim_run(pm_sim* sim, double a, double da, int n_steps, double* a_out);
 /* per-stage timing with HIP events on the sim's stream (0 off, 1 on); stage order:
- * deposit, r2c_x, c2c_y_fwd, z_green, c2c_y_inv, c2r_x, gather_kick_drift                   */
+ * deposit, r2c_x, c2c_y_fwd, z_green, c2c_y_inv, c2r_x, gather_kick_drift, bin              */
 PM_API int pm_sim_set_timing(pm_sim* sim, int enable);
 PM_API int pm_sim_get_timing(pm_sim* sim, double* stage_ms, int64_t* stage_calls, int n_stages);
 PM_API int pm_sim_sync(pm_sim* sim);
