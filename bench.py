@@ -171,11 +171,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a known-size kernel for "
+                         "calibrating FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); a multi-GPU PM step "
               "is not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy_mib:
+        from pmgpu.device import DeviceArray
+        nbytes = args.calibrate_copy_mib << 20
+        src = DeviceArray.zeros((nbytes // 4,), np.float32)
+        dst = DeviceArray.empty((nbytes // 4,), np.float32)
+        from pmgpu import _lib
+        for _ in range(3):
+            _lib.call("pm_memcpy_dtod", dst.ptr, src.ptr, nbytes)
+        log(f"calibration copy: 3 x {args.calibrate_copy_mib} MiB device-to-device")
+        del src, dst
     w = WORKLOAD
     n, n_p = w["n_cells"], w["n_parts"] ** 3
     log("initial conditions (CPU, seeded)")

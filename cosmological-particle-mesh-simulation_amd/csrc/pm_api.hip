@@ -1,4 +1,5 @@
 // libpm C ABI: stateless reference-function mirrors and the device-resident PM simulation.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -372,6 +373,11 @@ struct pm_sim {
   Bins B{};
   hipStream_t stream = nullptr;
   bool has_state = false, rho_valid = false, phi_valid = false, bins_valid = false;
+  // graph replay of consecutive steps: per-step scalars in a device table read at *d_step
+  double* d_table = nullptr;   // kTableSteps x {green coefficient, kick, drift}
+  int* d_step = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
   bool timing = false;
   struct Pending {
     int stage;
@@ -450,14 +456,41 @@ template <typename R> int sim_deposit_impl(pm_sim* s) {
   return PM_OK;
 }
 
-template <typename R> int sim_step_impl(pm_sim* s, double a, double da) {
+// per-step scalars of the step at scale factor a with step da
+struct StepScalars {
+  double green, kick, drift;
+};
+
+StepScalars step_scalars(const pm_sim* s, double a, double da) {
+  StepScalars c;
+  const double n3 = (double)s->L.n * s->L.n * s->L.n;
+  c.green = (-3 * s->omega_m0 / 8 / a) / n3;  // potential.py:15 with the 1/N^3 of the inverse DFT
+  // integrate.py:12 — f(a+da, [H0, OL, OK]) (Q1) unless the fix flag is set
+  const double c0 = (s->flags & PM_FLAG_FIX_F_COSMOLOGY) ? s->omega_m0 : s->h0;
+  const double fa1 = f_of_a(a + da, c0, s->omega_l0, s->omega_k0);
+  c.kick = da * fa1;                          // integrate.py:94
+  c.drift = da / ((a + da) * (a + da)) * fa1;  // integrate.py:95
+  return c;
+}
+
+// One PM step.  With `table` set the step's scalars are read on the device from
+// d_table[*d_step] and the counter is advanced at the end, so the enqueued work is the same
+// for every step and can be captured once into a graph.
+template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool table = false) {
   const int n = s->L.n;
-  if (!s->rho_valid || !s->bins_valid)
+  if (!s->rho_valid || !s->bins_valid) {
+    if (table) return fail(PM_E_STATE, "a captured step needs a valid density");
     if (int rc = sim_deposit_impl<R>(s)) return rc;
+  }
   cpx<R>* H = (cpx<R>*)s->H;
   cpx<R>* Q = (cpx<R>*)s->Q;
+  const StepScalars sc = step_scalars(s, a, da);
   GreenArgs g{};
-  g.coef = (-3 * s->omega_m0 / 8 / a) / ((double)n * n * n);  // potential.py:15 with 1/N^3
+  g.coef = sc.green;
+  if (table) {
+    g.table = s->d_table;
+    g.step = s->d_step;
+  }
   g.deconvolve = (s->flags & PM_FLAG_CIC_DECONVOLVE) ? 1 : 0;
   if (s->fgrid) {
     g.from_grid = 1;
@@ -484,14 +517,14 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da) {
     PM_HIP((fft_c2r_x<R, R>(H, Q, (R*)s->H, n, *s->tab, s->stream)));
   }
   s->phi_valid = true;
-  // integrate.py:12 — f(a+da, [H0, OL, OK]) (Q1) unless the fix flag is set
-  const double c0 = (s->flags & PM_FLAG_FIX_F_COSMOLOGY) ? s->omega_m0 : s->h0;
-  const double fa1 = f_of_a(a + da, c0, s->omega_l0, s->omega_k0);
-  const double kick = da * fa1;                          // integrate.py:94
-  const double drift = da / ((a + da) * (a + da)) * fa1;  // integrate.py:95
+  StepTable tab;
+  if (table) {
+    tab.table = s->d_table;
+    tab.step = s->d_step;
+  }
   {
     StageTimer t(s, ST_GKD);
-    PM_HIP(gkd_tiles<R>((R*)s->s, (R*)s->v, (const R*)s->H, kick, drift, s->L, s->G, s->B, s->stream));
+    PM_HIP(gkd_tiles<R>((R*)s->s, (R*)s->v, (const R*)s->H, sc.kick, sc.drift, s->L, s->G, s->B, tab, s->stream));
   }
   s->rho_valid = false;
   {
@@ -500,7 +533,71 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da) {
     PM_HIP(bins_scan(s->G, s->B, s->stream));
     PM_HIP(bins_scatter<R>((const R*)s->s, s->L, s->G, s->B, s->stream));
   }
-  return sim_deposit_impl<R>(s);
+  if (int rc = sim_deposit_impl<R>(s)) return rc;
+  if (table) PM_HIP(step_advance(s->d_step, s->stream));
+  return PM_OK;
+}
+
+constexpr int kTableSteps = 4096;  // steps per uploaded scalar table
+constexpr int kGraphSteps = 8;     // steps captured in one graph
+
+// capture kGraphSteps table-driven steps of this sim into an executable graph (once)
+template <typename R> int sim_build_graph(pm_sim* s) {
+  if (s->graph) return PM_OK;
+  if (!s->d_table) {
+    PM_HIP(hipMalloc(&s->d_table, sizeof(double) * 3 * kTableSteps));
+    PM_HIP(hipMalloc(&s->d_step, sizeof(int)));
+  }
+  hipGraph_t graph = nullptr;
+  PM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = PM_OK;
+  for (int k = 0; k < kGraphSteps && rc == PM_OK; ++k) rc = sim_step_impl<R>(s, 1.0, 0.0, true);
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (rc != PM_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(PM_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&s->graph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    s->graph = nullptr;
+    return fail(PM_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+  }
+  s->graph_steps = kGraphSteps;
+  return PM_OK;
+}
+
+// n steps from a through replays of the captured graph (remainder steps table-driven, direct)
+template <typename R> int sim_run_graph(pm_sim* s, double& a, double da, int n_steps) {
+  if (!s->rho_valid || !s->bins_valid)
+    if (int rc = sim_deposit_impl<R>(s)) return rc;
+  if (int rc = sim_build_graph<R>(s)) return rc;
+  std::vector<double> host(3 * (size_t)kTableSteps);
+  int done = 0;
+  while (done < n_steps) {
+    const int chunk = std::min(n_steps - done, kTableSteps);
+    for (int k = 0; k < chunk; ++k) {
+      if (!(a > 0.0)) return fail(PM_E_INVALID, "scale factor must be positive");
+      const StepScalars c = step_scalars(s, a, da);
+      host[3 * k] = c.green;
+      host[3 * k + 1] = c.kick;
+      host[3 * k + 2] = c.drift;
+      a += da;  // pmesh.py:63
+    }
+    const int zero = 0;
+    PM_HIP(hipMemcpyAsync(s->d_table, host.data(), sizeof(double) * 3 * chunk, hipMemcpyHostToDevice, s->stream));
+    PM_HIP(hipMemcpyAsync(s->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    int k = 0;
+    for (; k + s->graph_steps <= chunk; k += s->graph_steps) PM_HIP(hipGraphLaunch(s->graph, s->stream));
+    for (; k < chunk; ++k)
+      if (int rc = sim_step_impl<R>(s, 1.0, 0.0, true)) return rc;
+    // the host buffers must outlive the asynchronous copies above
+    PM_HIP(hipStreamSynchronize(s->stream));
+    done += chunk;
+  }
+  s->rho_valid = s->bins_valid = s->phi_valid = true;
+  return PM_OK;
 }
 
 }  // namespace
@@ -582,6 +679,9 @@ PM_API int pm_sim_destroy(pm_sim* s) {
   (void)hipFree(s->B.cursors);
   (void)hipFree(s->B.entries);
   (void)hipFree(s->B.block_sums);
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  (void)hipFree(s->d_table);
+  (void)hipFree(s->d_step);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return PM_OK;
@@ -599,13 +699,23 @@ PM_API int pm_sim_set_cosmology(pm_sim* s, double omega_m0, double omega_lambda0
 PM_API int pm_sim_set_mass(pm_sim* s, double mass) {
   if (!s) return fail(PM_E_INVALID, "null sim");
   if (!(mass > 0.0)) return fail(PM_E_INVALID, "mass must be positive");
-  if (mass != s->mass) s->rho_valid = false;
+  if (mass != s->mass) {
+    s->rho_valid = false;
+    if (s->graph) {  // the captured deposit holds the old mass
+      (void)hipGraphExecDestroy(s->graph);
+      s->graph = nullptr;
+    }
+  }
   s->mass = mass;
   return PM_OK;
 }
 
 PM_API int pm_sim_set_green(pm_sim* s, const float* fgrid) {
   if (!s) return fail(PM_E_INVALID, "null sim");
+  if (fgrid != s->fgrid && s->graph) {  // the captured k-space pass holds the old grid
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+  }
   s->fgrid = fgrid;
   return PM_OK;
 }
@@ -777,6 +887,12 @@ PM_API int pm_sim_run(pm_sim* s, double a, double da, int n_steps, double* a_out
   if (!s) return fail(PM_E_INVALID, "null sim");
   if (!s->has_state) return fail(PM_E_STATE, "state was never set");
   if (n_steps < 0) return fail(PM_E_INVALID, "n_steps must be >= 0");
+  if (!s->timing && !(s->flags & PM_FLAG_NO_GRAPH) && n_steps >= 2) {
+    int rc = s->prec == 4 ? sim_run_graph<float>(s, a, da, n_steps) : sim_run_graph<double>(s, a, da, n_steps);
+    if (rc) return rc;
+    if (a_out) *a_out = a;
+    return PM_OK;
+  }
   for (int k = 0; k < n_steps; ++k) {
     if (!(a > 0.0)) return fail(PM_E_INVALID, "scale factor must be positive");
     int rc = s->prec == 4 ? sim_step_impl<float>(s, a, da) : sim_step_impl<double>(s, a, da);

@@ -151,6 +151,15 @@ struct GreenArgs {
   int from_grid;        // 0: analytic 1/sum sin^2(k/2); 1: from a float32 N^3 grid
   int deconvolve;       // divide by the CIC window of deposit and gather (prod sinc^4)
   const float* fgrid;   // dense [z][y][x] float32 grid when from_grid = 1
+  // graph replay: when set, coef is table[3 * (*step) + 0] (the host value is ignored)
+  const double* table = nullptr;
+  const int* step = nullptr;
+};
+
+// per-step scalars of a captured step: {green coefficient, kick, drift} per step
+struct StepTable {
+  const double* table = nullptr;
+  const int* step = nullptr;
 };
 
 // per-size constant tables held in device memory

@@ -212,16 +212,16 @@ struct PencilSeg {
 };
 
 template <typename R>
-__device__ __forceinline__ R green_factor(int kx, int ky, int kz, int n, const GreenArgs& g, const R* s2,
+__device__ __forceinline__ R green_factor(int kx, int ky, int kz, int n, const GreenArgs& g, R coef, const R* s2,
                                           const R* wd) {
   if (g.from_grid) {
     const size_t nn = (size_t)n;
     const size_t i1 = ((size_t)kz * nn + ky) * nn + kx;
     const size_t i2 = ((size_t)((n - kz) % n) * nn + (n - ky) % n) * nn + (n - kx) % n;
-    return R(g.coef) * R(0.5) * (R(g.fgrid[i1]) + R(g.fgrid[i2]));
+    return coef * R(0.5) * (R(g.fgrid[i1]) + R(g.fgrid[i2]));
   }
   const R den = s2[kx] + s2[ky] + s2[kz];
-  R f = den > R(0) ? R(g.coef) / den : R(0);
+  R f = den > R(0) ? coef / den : R(0);
   if (g.deconvolve) f *= wd[kx] * wd[ky] * wd[kz];
   return f;
 }
@@ -308,10 +308,11 @@ __global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, Penci
     fft_passes<T, L, 1, +1, W, SN, NT, 1>(lds, tw, tid);
   } else {
     fft_passes<T, L, 1, -1, W, SN, NT, 1>(lds, tw, tid);
+    const R coef = R(g.table ? g.table[3 * (*g.step)] : g.coef);
 #pragma unroll 4
     for (int e = tid; e < L * W; e += NT) {
       const int n = e / W, w = e - n * W;
-      const R f = green_factor<R>(kx0 + w * dkx, ky0 + w * dky, n, N, g, s2, wd);
+      const R f = green_factor<R>(kx0 + w * dkx, ky0 + w * dky, n, N, g, coef, s2, wd);
       lds[n * SN + w] = cscale(lds[n * SN + w], f);
     }
     __syncthreads();

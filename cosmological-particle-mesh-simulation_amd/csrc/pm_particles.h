@@ -2,6 +2,7 @@
 // and the reference-layout kernels behind the stateless C-ABI entry points.
 #pragma once
 #include "pm_common.h"
+#include "pm_fft.h"
 
 namespace pm {
 
@@ -111,7 +112,9 @@ hipError_t deposit_tiles(const R* s, R* rho, double mass, const Lattice& L, cons
 // also counts the entries of the new positions into B.counts
 template <typename R>
 hipError_t gkd_tiles(R* s, R* v, const R* phi, double kick, double drift, const Lattice& L, const TileGeom& G,
-                     const Bins& B, hipStream_t st);
+                     const Bins& B, const StepTable& tab, hipStream_t st);
+// *step += 1 on the device (advances a captured step's scalar table)
+hipError_t step_advance(int* step, hipStream_t st);
 
 // ---- reference-layout kernels (absolute float32 positions, reference order) ------------------
 hipError_t ref_density(const float* pos, long np, int n, double mass, float* rho, hipStream_t st);

@@ -31,6 +31,17 @@ inline unsigned blocks_for(long n, int b = kT) {
 // index in [-n, 2n) folded into [0, n)
 __device__ __forceinline__ int fold(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// XCD-aware block -> tile map.  Consecutive workgroups are dealt round-robin to the 8 XCDs,
+// each with its own L2; this hands every XCD one contiguous run of tiles instead, so tiles
+// that share cache lines (x and y neighbours of the potential box, particles straddling a
+// tile face) are processed by the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+  const int per = nblocks / kXcds, rem = nblocks % kXcds;
+  const int xcd = b % kXcds, local = b / kXcds;
+  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
 // canonical position of a resident particle from its displacement: cell, CIC fractions and
 // lattice point q per axis
 template <typename R>
@@ -246,8 +257,7 @@ __global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R
   // banks instead of sharing the few that a power-of-two pitch leaves them
   constexpr int PX = TX + 3, PLANE = PX * TY, PADDED = PLANE * TZ;
   __shared__ double acc[PADDED];
-  for (int i = threadIdx.x; i < PADDED; i += kT) acc[i] = 0.0;
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tix = tile % G.ntx, tiy = (tile / G.ntx) % G.nty, tiz = tile / (G.ntx * G.nty);
   const int o0[3] = {tix * TX, tiy * TY, tiz * TZ};
   const int T[3] = {TX, TY, TZ};
@@ -255,19 +265,24 @@ __global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R
   const long np = L.np;
   const uint32_t end = min(offsets[2 * tile + 2], capacity);
   const uint32_t beg = min(offsets[2 * tile], end);
-  __syncthreads();
-  for (uint32_t base = beg; base < end; base += kT * E) {
-    uint32_t sl[E];
+  uint32_t sl[E];
+  R sv[E][3];
+  auto load = [&](uint32_t base) {
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const uint32_t e = base + u * kT + threadIdx.x;
       sl[u] = e < end ? min(entries[e], (uint32_t)(np - 1)) : kNone;
     }
-    R sv[E][3];
 #pragma unroll
     for (int u = 0; u < E; ++u)
 #pragma unroll
       for (int a = 0; a < 3; ++a) sv[u][a] = sl[u] != kNone ? s[a * np + sl[u]] : R(0);
+  };
+  load(beg);  // the first batch is in flight while the accumulator is cleared
+  for (int i = threadIdx.x; i < PADDED; i += kT) acc[i] = 0.0;
+  __syncthreads();
+  for (uint32_t base = beg; base < end; base += kT * E) {
+    if (base != beg) load(base);
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       if (sl[u] == kNone) continue;
@@ -329,12 +344,12 @@ __global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R
 // the new position are counted for the next binning.  The potential box is fetched in
 // batches of CH loads per thread before they are written to LDS.
 // ------------------------------------------------------------------------------------------
-template <typename R, int TX, int TY, int TZ>
+template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
 __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi,
                                                   R kick, R drift, Lattice L, TileGeom G,
                                                   const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
-                                                  uint32_t capacity) {
+                                                  uint32_t capacity, StepTable tab) {
   constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, BN = BX * BY * BZ;
   static_assert(BX <= 64, "a box row must fit in one wavefront");
   constexpr int RPW = 64 / BX;                      // box rows per wavefront load
@@ -344,7 +359,7 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
   constexpr int E = 2;
   __shared__ R box[BN];
   __shared__ uint32_t hist[128];  // new-tile counts: offset (-1..2)^3 from this tile x {own, halo}
-  const int tile = blockIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t end = min(offsets[2 * tile + 1], capacity);
   const uint32_t beg = min(offsets[2 * tile], end);
   if (beg == end) return;  // empty tile: the whole block leaves together
@@ -353,9 +368,31 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
   const int n = L.n;
   const long np = L.np;
   const size_t nn = (size_t)n;
-  // the local histogram needs 4 distinct tiles per axis to name the neighbourhood uniquely
-  const bool local_hist = G.ntx >= 4 && G.nty >= 4 && G.ntz >= 4;
-  if (threadIdx.x < 128) hist[threadIdx.x] = 0u;
+  // LOCAL_HIST: counts of the new tiles go through an LDS histogram of the 4x4x4 tile
+  // neighbourhood, which needs 4 distinct tiles per axis to name it uniquely (the launcher
+  // checks); otherwise each wavefront aggregates its own atomics
+  if (LOCAL_HIST && threadIdx.x < 128) hist[threadIdx.x] = 0u;
+  // The first batch of entries is fetched before the potential box, the batch's particle
+  // state together with the box, so the two dependent load chains overlap.
+  uint32_t sl[E];
+  R sv[E][3], vv[E][3];
+  auto load_entries = [&](uint32_t base) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const uint32_t e = base + u * kT + threadIdx.x;
+      sl[u] = e < end ? min(entries[e], (uint32_t)(np - 1)) : kNone;
+    }
+  };
+  auto load_state = [&]() {
+#pragma unroll
+    for (int u = 0; u < E; ++u)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        sv[u][a] = sl[u] != kNone ? s[a * np + sl[u]] : R(0);
+        vv[u][a] = sl[u] != kNone ? v[a * np + sl[u]] : R(0);
+      }
+  };
+  load_entries(beg);
   // potential of the tile and its -1/+2 stencil margin (periodic), one box row per lane group
   {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -372,6 +409,7 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
         tmp[it] = phi[((size_t)gz * nn + gy) * nn + gx];
       }
     }
+    load_state();
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
       const int row = RPW * (wv + NW * it) + rl;
@@ -380,21 +418,16 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
   }
   __syncthreads();
   const R nr = R(n);
+  if (tab.table) {  // graph replay: this step's kick and drift from the device table
+    const int k = *tab.step;
+    kick = R(tab.table[3 * k + 1]);
+    drift = R(tab.table[3 * k + 2]);
+  }
   for (uint32_t base = beg; base < end; base += kT * E) {
-    uint32_t sl[E];
-#pragma unroll
-    for (int u = 0; u < E; ++u) {
-      const uint32_t e = base + u * kT + threadIdx.x;
-      sl[u] = e < end ? min(entries[e], (uint32_t)(np - 1)) : kNone;
+    if (base != beg) {
+      load_entries(base);
+      load_state();
     }
-    R sv[E][3], vv[E][3];
-#pragma unroll
-    for (int u = 0; u < E; ++u)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        sv[u][a] = sl[u] != kNone ? s[a * np + sl[u]] : R(0);
-        vv[u][a] = sl[u] != kNone ? v[a * np + sl[u]] : R(0);
-      }
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const bool valid = sl[u] != kNone;
@@ -403,15 +436,18 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
         int c[3];
         R d[3], q[3];
         cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], sl[u], L, c, d, q);
+        // tile-relative cell (box coordinate of the cell is that + 1).  Own entries were binned
+        // from these same positions, so the cell lies in the tile; the clamp only keeps a
+        // corrupted state inside the box.
         int b[3];
-        bool inside = true;
+        const int T[3] = {TX, TY, TZ};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           int rel = c[a] - o0[a];
           if (rel < 0) rel += n;
-          b[a] = rel;  // box coordinate of the cell is rel + 1
-          inside = inside && rel < (a == 0 ? TX : (a == 1 ? TY : TZ));
+          b[a] = rel < T[a] ? rel : T[a] - 1;
         }
+        const R* bp = box + (b[2] * BY + b[1]) * BX + b[0];
         R P[4][4][4];
 #pragma unroll
         for (int zz = 0; zz < 4; ++zz)
@@ -420,13 +456,7 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
 #pragma unroll
             for (int xx = 0; xx < 4; ++xx) {
               const int outside = (xx == 0 || xx == 3) + (yy == 0 || yy == 3) + (zz == 0 || zz == 3);
-              if (outside <= 1) {
-                if (inside)
-                  P[zz][yy][xx] = box[((b[2] + zz) * BY + (b[1] + yy)) * BX + (b[0] + xx)];
-                else  // a particle outside its own tile: inconsistent bins, read the grid itself
-                  P[zz][yy][xx] =
-                      phi[((size_t)wrapi(c[2] + zz - 1, n) * nn + wrapi(c[1] + yy - 1, n)) * nn + wrapi(c[0] + xx - 1, n)];
-              }
+              if (outside <= 1) P[zz][yy][xx] = bp[(zz * BY + yy) * BX + xx];
             }
         const R w0[2] = {R(1) - d[0], d[0]};
         const R w1[2] = {R(1) - d[1], d[1]};
@@ -453,7 +483,7 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
           cn[a] = cell_of<R>(canonical_pos<R>(q[a], sa, nr), n);
         }
         tile_of<TX, TY, TZ>(cn, G, tn, up);
-        if (local_hist) {
+        if (LOCAL_HIST) {
           const int tc[3] = {tix, tiy, tiz};
           const int ntc[3] = {G.ntx, G.nty, G.ntz};
 #pragma unroll 1
@@ -475,7 +505,7 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
           }
         }
       }
-      if (!local_hist) {
+      if (!LOCAL_HIST) {
 #pragma unroll 1
         for (int o = 0; o < 8; ++o) {
           const bool act = valid && target_active(up, o);
@@ -484,7 +514,7 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
       }
     }
   }
-  if (local_hist) {
+  if (LOCAL_HIST) {
     __syncthreads();
     if (threadIdx.x < 128) {
       const uint32_t cnt = hist[threadIdx.x];
@@ -498,7 +528,16 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
   }
 }
 
+__global__ void k_step_advance(int* step) {
+  if (threadIdx.x == 0) *step += 1;
+}
+
 }  // namespace
+
+hipError_t step_advance(int* step, hipStream_t st) {
+  k_step_advance<<<1, 64, 0, st>>>(step);
+  return hipGetLastError();
+}
 
 TileGeom make_tiles(int n, int prec_bytes) {
   auto pick = [n](int pref) {
@@ -524,10 +563,8 @@ int scan_blocks(const TileGeom& g) { return (int)((2L * g.ntiles + kScanChunk - 
 // tile shapes the kernels are instantiated for; make_tiles only produces these
 #define PM_TILE_DISPATCH(G, CALL)                                        \
   do {                                                                   \
-    if ((G).tx == 32 && (G).ty == 16 && (G).tz == 16) CALL(32, 16, 16);  \
-    else if ((G).tx == 16 && (G).ty == 16 && (G).tz == 16) CALL(16, 16, 16); \
+    if ((G).tx == 16 && (G).ty == 16 && (G).tz == 16) CALL(16, 16, 16);      \
     else if ((G).tx == 8 && (G).ty == 8 && (G).tz == 8) CALL(8, 8, 8);   \
-    else if ((G).tx == 16 && (G).ty == 8 && (G).tz == 8) CALL(16, 8, 8); \
     else return hipErrorInvalidValue;                                    \
   } while (0)
 
@@ -569,10 +606,17 @@ hipError_t deposit_tiles(const R* s, R* rho, double mass, const Lattice& L, cons
 
 template <typename R>
 hipError_t gkd_tiles(R* s, R* v, const R* phi, double kick, double drift, const Lattice& L, const TileGeom& G,
-                     const Bins& B, hipStream_t st) {
-#define PM_L(X, Y, Z)                                                                                       \
-  k_gkd_tiles<R, X, Y, Z><<<G.ntiles, kT, 0, st>>>(s, v, phi, R(kick), R(drift), L, G, B.offsets, B.entries, \
-                                                   B.counts, B.capacity)
+                     const Bins& B, const StepTable& tab, hipStream_t st) {
+  const bool local_hist = G.ntx >= 4 && G.nty >= 4 && G.ntz >= 4;
+#define PM_L(X, Y, Z)                                                                                            \
+  do {                                                                                                           \
+    if (local_hist)                                                                                              \
+      k_gkd_tiles<R, X, Y, Z, true><<<G.ntiles, kT, 0, st>>>(s, v, phi, R(kick), R(drift), L, G, B.offsets,      \
+                                                             B.entries, B.counts, B.capacity, tab);              \
+    else                                                                                                         \
+      k_gkd_tiles<R, X, Y, Z, false><<<G.ntiles, kT, 0, st>>>(s, v, phi, R(kick), R(drift), L, G, B.offsets,     \
+                                                              B.entries, B.counts, B.capacity, tab);             \
+  } while (0)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
   return hipGetLastError();
@@ -584,7 +628,7 @@ hipError_t gkd_tiles(R* s, R* v, const R* phi, double kick, double drift, const 
   template hipError_t deposit_tiles<R>(const R*, R*, double, const Lattice&, const TileGeom&, const Bins&,   \
                                        hipStream_t);                                                         \
   template hipError_t gkd_tiles<R>(R*, R*, const R*, double, double, const Lattice&, const TileGeom&,        \
-                                   const Bins&, hipStream_t);
+                                   const Bins&, const StepTable&, hipStream_t);
 PM_INST(float)
 PM_INST(double)
 #undef PM_INST

@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpm.so"
 PM_OK = 0
 PM_FLAG_CIC_DECONVOLVE = 1
 PM_FLAG_FIX_F_COSMOLOGY = 2
+PM_FLAG_NO_GRAPH = 4
 
 # N stages timed by pm_sim_get_timing, in order
 STAGES = ("deposit", "r2c_x", "c2c_y_fwd", "z_green", "c2c_y_inv", "c2r_x", "gather_kick_drift",

@@ -20,7 +20,7 @@ from . import _lib
 
 class PMSim:
     def __init__(self, n_cells: int, n_parts: int, precision: int = 32, *,
-                 cic_deconvolve: bool = False, fix_f_cosmology: bool = False,
+                 cic_deconvolve: bool = False, fix_f_cosmology: bool = False, graph: bool = True,
                  omega_m0: float = 0.31, omega_lambda0: float = 0.69, omega_k0: float = 0.0,
                  h0: float = 0.68):
         _lib.init()
@@ -33,6 +33,8 @@ class PMSim:
             flags |= _lib.PM_FLAG_CIC_DECONVOLVE
         if fix_f_cosmology:
             flags |= _lib.PM_FLAG_FIX_F_COSMOLOGY
+        if not graph:
+            flags |= _lib.PM_FLAG_NO_GRAPH
         h = C.c_void_p()
         _lib.call("pm_sim_create", self.n_cells, self.n_parts, self.precision, flags, C.byref(h))
         self._h = h

@@ -37,6 +37,8 @@ extern "C" {
 /* sim flags */
 #define PM_FLAG_CIC_DECONVOLVE 1u   /* divide by the CIC window of deposit and gather        */
 #define PM_FLAG_FIX_F_COSMOLOGY 2u  /* f() gets [Omega_m, OL, OK] instead of [H0, OL, OK]     */
+#define PM_FLAG_NO_GRAPH 4u         /* pm_sim_run enqueues every kernel instead of replaying a
+                                       captured graph of consecutive steps                     */
 
 /* ---------------------------------------------------------------- library / device memory */
 PM_API int pm_init(int device);                 /* select device, create the library stream */
@@ -117,7 +119,9 @@ PM_API int pm_sim_get_density(pm_sim* sim, void* rho, int dtype_bytes, int on_de
 PM_API int pm_sim_get_potential(pm_sim* sim, void* phi, int dtype_bytes, int on_device);
 /* one PM step: pmesh.py:60-61 (density + advance_time) at scale factor a with step da.     */
 PM_API int pm_sim_step(pm_sim* sim, double a, double da);
-/* n_steps steps from a with step da, enqueued back to back; returns the final a.            */
+/* n_steps steps from a with step da; returns the final a.  Unless PM_FLAG_NO_GRAPH or timing
+ * is set, the steps replay one captured graph of consecutive steps whose scalars (Green's
+ * coefficient, kick, drift) come from a device table filled for the whole call.             */
 PM_API int pm_sim_run(pm_sim* sim, double a, double da, int n_steps, double* a_out);
 /* per-stage timing with HIP events on the sim's stream (0 off, 1 on); stage order:
  * deposit, r2c_x, c2c_y_fwd, z_green, c2c_y_inv, c2r_x, gather_kick_drift, bin              */

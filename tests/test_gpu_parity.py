@@ -238,6 +238,29 @@ def test_sim_run_equals_repeated_step():
     np.testing.assert_allclose(v1, v2, rtol=1e-5, atol=1e-6 * np.abs(v1).max())
 
 
+def test_graph_replays_match_eager_steps():
+    # the captured step graph is replayed several times, with eager work and synchronisation
+    # between replays; every replay must agree with the same steps enqueued kernel by kernel
+    n, npart = 64, 32
+    pos, vel = make_ics(n, npart, seed=23)
+    g, e = _sim(n, npart, 32), _sim(n, npart, 32, graph=False)
+    for s in (g, e):
+        s.set_state(pos, vel)
+    a, da = 0.02, 0.0098
+    for rep in range(4):
+        a_g = g.run(a, da, 9)          # one replay of 8 captured steps + 1 table-driven step
+        a_e = e.run(a, da, 9)
+        assert a_g == a_e
+        xg, vg = g.get_state()          # eager work between replays
+        xe, ve = e.get_state()
+        assert np.abs(periodic_diff(xg, xe, n)).max() < 1e-4, rep
+        assert rel(vg, ve) < 1e-5, rep
+        assert rel(g.density(np.float64), e.density(np.float64)) < 1e-6, rep
+        a = a_g
+    for s in (g, e):
+        s.close()
+
+
 def test_sim_deconvolve_and_fix_flags_change_result():
     n, npart = 32, 16
     pos, vel = make_ics(n, npart, seed=21)
