@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/pm.h"
+#include "pm_batch.h"
 #include "pm_fft.h"
 #include "pm_particles.h"
 
@@ -356,6 +357,12 @@ PM_API double pm_f(double a, double c0, double c1, double c2) { return f_of_a(a,
 // ------------------------------------------------------------------------------------------
 // device-resident simulation
 // ------------------------------------------------------------------------------------------
+// HIP event pair around one timed stage
+struct PendingEvent {
+  int stage;
+  hipEvent_t a, b;
+};
+
 struct pm_sim {
   Lattice L{};
   int prec = 4;
@@ -379,13 +386,10 @@ struct pm_sim {
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
   bool timing = false;
-  struct Pending {
-    int stage;
-    hipEvent_t a, b;
-  };
-  std::vector<Pending> pending;
+  std::vector<PendingEvent> pending;
   double stage_ms[ST_COUNT] = {};
   int64_t stage_calls[ST_COUNT] = {};
+  pm_ensemble* ens = nullptr;  // ensemble this sim is a member of
 };
 
 namespace {
@@ -402,26 +406,29 @@ size_t sim_bytes(int n, int np_side, int prec) {
          bins_bytes(n, np_side, prec);
 }
 
-struct StageTimer {
-  pm_sim* s;
+// records HIP events around a stage on the owner's stream when its timing is on (the owner is
+// a sim or an ensemble)
+template <class O> struct StageTimerT {
+  O* s;
   int stage;
   hipEvent_t a = nullptr, b = nullptr;
-  StageTimer(pm_sim* sim, int st) : s(sim), stage(st) {
+  StageTimerT(O* sim, int st) : s(sim), stage(st) {
     if (s->timing) {
       (void)hipEventCreate(&a);
       (void)hipEventCreate(&b);
       (void)hipEventRecord(a, s->stream);
     }
   }
-  ~StageTimer() {
+  ~StageTimerT() {
     if (s->timing) {
       (void)hipEventRecord(b, s->stream);
       s->pending.push_back({stage, a, b});
     }
   }
 };
+using StageTimer = StageTimerT<pm_sim>;
 
-int sim_collect_timing(pm_sim* s) {
+template <class O> int collect_timing(O* s) {
   if (s->pending.empty()) return PM_OK;
   PM_HIP(hipStreamSynchronize(s->stream));
   for (auto& p : s->pending) {
@@ -435,6 +442,7 @@ int sim_collect_timing(pm_sim* s) {
   s->pending.clear();
   return PM_OK;
 }
+int sim_collect_timing(pm_sim* s) { return collect_timing(s); }
 
 // bin every particle into its tile entries from the current state
 template <typename R> int sim_bin_impl(pm_sim* s) {
@@ -664,6 +672,7 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
 
 PM_API int pm_sim_destroy(pm_sim* s) {
   if (!s) return PM_OK;
+  if (s->ens) (void)pm_ensemble_remove(s->ens, s);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto& p : s->pending) {
     (void)hipEventDestroy(p.a);
@@ -928,6 +937,365 @@ PM_API int pm_sim_get_timing(pm_sim* s, double* ms, int64_t* calls, int n_stages
 PM_API int pm_sim_sync(pm_sim* s) {
   if (!s) return fail(PM_E_INVALID, "null sim");
   PM_HIP(hipStreamSynchronize(s->stream));
+  return PM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// batched ensemble
+// ------------------------------------------------------------------------------------------
+struct pm_ensemble {
+  std::vector<pm_sim*> members;  // in the order of adding
+  std::vector<int> order;        // descriptor slot -> member index (size class 0 first)
+  BatchDesc* host = nullptr;     // host image of the descriptor
+  BatchDesc* d_desc = nullptr;
+  double* d_table = nullptr;     // kEnsTableSteps x kBatchMax x {Green, kick, drift}
+  int* d_step = nullptr;
+  std::vector<double> table;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+  bool shape_dirty = true;  // membership changed: launch shapes and the graph are stale
+  // launch shapes: FFT stages per size class, tile stages over all members
+  long fft_base[2][BS_COUNT] = {}, fft_count[2][BS_COUNT] = {};
+  int fft_lds[2][BS_COUNT] = {};
+  long total[BS_COUNT] = {};
+  bool timing = false;
+  std::vector<PendingEvent> pending;
+  double stage_ms[ST_COUNT] = {};
+  int64_t stage_calls[ST_COUNT] = {};
+};
+
+namespace {
+
+constexpr int kEnsTableSteps = 1024;  // steps per uploaded ensemble scalar table
+
+using EnsTimer = StageTimerT<pm_ensemble>;
+
+void ens_drop_graph(pm_ensemble* e) {
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  e->graph = nullptr;
+  e->graph_steps = 0;
+}
+
+// descriptor of the current members (host image) and the launch shapes
+void ens_describe(pm_ensemble* e) {
+  const int nm = (int)e->members.size();
+  e->order.resize(nm);
+  for (int i = 0; i < nm; ++i) e->order[i] = i;
+  std::stable_sort(e->order.begin(), e->order.end(), [e](int x, int y) {
+    return batch_class(e->members[x]->L.n) < batch_class(e->members[y]->L.n);
+  });
+  BatchDesc& D = *e->host;
+  std::memset(&D, 0, sizeof(BatchDesc));
+  D.n_members = nm;
+  D.table = e->d_table;
+  D.step = e->d_step;
+  for (int st = 0; st < BS_COUNT; ++st) D.off[st][0] = 0;
+  for (int c = 0; c < 2; ++c)
+    for (int st = 0; st < BS_COUNT; ++st) {
+      e->fft_base[c][st] = -1;
+      e->fft_count[c][st] = 0;
+      e->fft_lds[c][st] = 0;
+    }
+  for (int i = 0; i < nm; ++i) {
+    const pm_sim* s = e->members[e->order[i]];
+    BatchMember& m = D.m[i];
+    const int n = s->L.n;
+    m.rho = (float*)s->rho;
+    m.H = (cpx<float>*)s->H;
+    m.Q = (cpx<float>*)s->Q;
+    m.tw = (const cpx<float>*)s->tab->tw;
+    m.sin2 = (const float*)s->tab->sin2;
+    m.wdec = (const float*)s->tab->wdec;
+    batch_fft_segments(n, m.ya, m.yb, m.za, m.zb);
+    m.n = n;
+    m.deconvolve = (s->flags & PM_FLAG_CIC_DECONVOLVE) ? 1 : 0;
+    m.from_grid = s->fgrid ? 1 : 0;
+    m.fgrid = s->fgrid;
+    m.local_hist = (s->G.ntx >= 4 && s->G.nty >= 4 && s->G.ntz >= 4) ? 1 : 0;
+    m.s = (float*)s->s;
+    m.v = (float*)s->v;
+    m.mass = (float)s->mass;
+    m.capacity = s->B.capacity;
+    m.L = s->L;
+    m.G = s->G;
+    m.counts = s->B.counts;
+    m.offsets = s->B.offsets;
+    m.cursors = s->B.cursors;
+    m.entries = s->B.entries;
+    m.block_sums = s->B.block_sums;
+    m.scan_nb = scan_blocks(s->G);
+    D.max_n = std::max(D.max_n, n);
+    long blocks[BS_COUNT];
+    blocks[BS_R2C] = batch_fft_blocks(n, BS_R2C);
+    blocks[BS_Y] = batch_fft_blocks(n, BS_Y);
+    blocks[BS_Z] = batch_fft_blocks(n, BS_Z);
+    blocks[BS_C2R] = batch_fft_blocks(n, BS_C2R);
+    blocks[BS_TILES] = s->G.ntiles;
+    blocks[BS_SCAN] = m.scan_nb;
+    blocks[BS_SCATTER] = batch_scatter_blocks(s->L.np);
+    const int c = batch_class(n);
+    for (int st = 0; st < BS_COUNT; ++st) {
+      D.off[st][i + 1] = D.off[st][i] + blocks[st];
+      if (st <= BS_C2R) {
+        if (e->fft_base[c][st] < 0) e->fft_base[c][st] = D.off[st][i];
+        e->fft_count[c][st] += blocks[st];
+        e->fft_lds[c][st] = std::max(e->fft_lds[c][st], batch_fft_lds(n, st));
+      }
+    }
+  }
+  for (int st = 0; st < BS_COUNT; ++st) e->total[st] = D.off[st][nm];
+}
+
+// one table-driven step of every member (the enqueued work is identical for every step)
+int ens_step(pm_ensemble* e) {
+  const BatchDesc* d = e->d_desc;
+  const hipStream_t st = e->stream;
+  auto fft = [&](int stage, int mode) -> hipError_t {
+    for (int c = 0; c < 2; ++c) {
+      const long cnt = e->fft_count[c][stage];
+      if (cnt <= 0) continue;
+      hipError_t r;
+      const long base = e->fft_base[c][stage];
+      const int lds = e->fft_lds[c][stage];
+      if (stage == BS_R2C) r = batch_r2c(d, c, base, cnt, lds, st);
+      else if (stage == BS_C2R) r = batch_c2r(d, c, base, cnt, lds, st);
+      else r = batch_pencil(d, c, mode, base, cnt, lds, st);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  };
+  {
+    EnsTimer t(e, ST_R2C);
+    PM_HIP(fft(BS_R2C, 0));
+  }
+  {
+    EnsTimer t(e, ST_Y_FWD);
+    PM_HIP(fft(BS_Y, -1));
+  }
+  {
+    EnsTimer t(e, ST_Z_GREEN);
+    PM_HIP(fft(BS_Z, 0));
+  }
+  {
+    EnsTimer t(e, ST_Y_INV);
+    PM_HIP(fft(BS_Y, +1));
+  }
+  {
+    EnsTimer t(e, ST_C2R);
+    PM_HIP(fft(BS_C2R, 0));
+  }
+  {
+    EnsTimer t(e, ST_GKD);
+    PM_HIP(batch_gkd(d, e->total[BS_TILES], st));
+  }
+  {
+    EnsTimer t(e, ST_BIN);
+    PM_HIP(batch_scan(d, e->total[BS_SCAN], (int)e->members.size(), st));
+    PM_HIP(batch_scatter(d, e->total[BS_SCATTER], st));
+  }
+  {
+    EnsTimer t(e, ST_DEPOSIT);
+    PM_HIP(batch_deposit(d, e->total[BS_TILES], st));
+  }
+  PM_HIP(step_advance(e->d_step, st));
+  return PM_OK;
+}
+
+int ens_build_graph(pm_ensemble* e) {
+  if (e->graph) return PM_OK;
+  hipGraph_t graph = nullptr;
+  PM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  int rc = PM_OK;
+  for (int k = 0; k < kGraphSteps && rc == PM_OK; ++k) rc = ens_step(e);
+  hipError_t err = hipStreamEndCapture(e->stream, &graph);
+  if (rc != PM_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (err != hipSuccess) return fail(PM_E_HIP, "ensemble graph capture failed: %s", hipGetErrorString(err));
+  err = hipGraphInstantiate(&e->graph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (err != hipSuccess) {
+    e->graph = nullptr;
+    return fail(PM_E_HIP, "ensemble graph instantiation failed: %s", hipGetErrorString(err));
+  }
+  e->graph_steps = kGraphSteps;
+  return PM_OK;
+}
+
+int ens_check_member(const pm_sim* s) {
+  if (s->prec != 4) return fail(PM_E_UNSUPPORTED, "ensemble members must be FP32 sims");
+  if (!batch_size_supported(s->L.n))
+    return fail(PM_E_UNSUPPORTED, "N_CELLS=%d is not a batched grid size (32..512, 2^a 3^b, multiple of 16)",
+                s->L.n);
+  if (s->G.tx != 16 || s->G.ty != 16 || s->G.tz != 16)
+    return fail(PM_E_UNSUPPORTED, "ensemble members need 16^3 tiles");
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+PM_API int pm_ensemble_create(pm_ensemble** out) {
+  PM_INIT();
+  if (!out) return fail(PM_E_INVALID, "null output handle");
+  *out = nullptr;
+  static std::once_flag once;
+  static hipError_t init_err = hipSuccess;
+  std::call_once(once, [] { init_err = batch_fft_init(); });
+  if (init_err != hipSuccess) return fail(PM_E_HIP, "batched FFT set-up failed: %s", hipGetErrorString(init_err));
+  pm_ensemble* e = new pm_ensemble();
+  e->host = new BatchDesc();
+  if (hipMalloc(&e->d_desc, sizeof(BatchDesc)) != hipSuccess ||
+      hipMalloc(&e->d_table, sizeof(double) * 3 * kBatchMax * kEnsTableSteps) != hipSuccess ||
+      hipMalloc(&e->d_step, sizeof(int)) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess) {
+    pm_ensemble_destroy(e);
+    return fail(PM_E_NOMEM, "ensemble allocation failed");
+  }
+  e->table.resize((size_t)3 * kBatchMax * kEnsTableSteps);
+  *out = e;
+  return PM_OK;
+}
+
+PM_API int pm_ensemble_destroy(pm_ensemble* e) {
+  if (!e) return PM_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (pm_sim* s : e->members) s->ens = nullptr;
+  for (auto& p : e->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  ens_drop_graph(e);
+  (void)hipFree(e->d_desc);
+  (void)hipFree(e->d_table);
+  (void)hipFree(e->d_step);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e->host;
+  delete e;
+  return PM_OK;
+}
+
+PM_API int pm_ensemble_add(pm_ensemble* e, pm_sim* s) {
+  if (!e || !s) return fail(PM_E_INVALID, "null argument");
+  if (s->ens == e) return fail(PM_E_INVALID, "the sim is already a member");
+  if (s->ens) return fail(PM_E_STATE, "the sim is a member of another ensemble");
+  if ((int)e->members.size() >= kBatchMax) return fail(PM_E_UNSUPPORTED, "an ensemble holds at most %d sims", kBatchMax);
+  if (int rc = ens_check_member(s)) return rc;
+  PM_HIP(hipStreamSynchronize(e->stream));
+  e->members.push_back(s);
+  s->ens = e;
+  e->shape_dirty = true;
+  return PM_OK;
+}
+
+PM_API int pm_ensemble_remove(pm_ensemble* e, pm_sim* s) {
+  if (!e || !s) return fail(PM_E_INVALID, "null argument");
+  auto it = std::find(e->members.begin(), e->members.end(), s);
+  if (it == e->members.end()) return fail(PM_E_INVALID, "the sim is not a member");
+  PM_HIP(hipStreamSynchronize(e->stream));
+  e->members.erase(it);
+  s->ens = nullptr;
+  e->shape_dirty = true;
+  return PM_OK;
+}
+
+PM_API int pm_ensemble_size(pm_ensemble* e, int* n) {
+  if (!e || !n) return fail(PM_E_INVALID, "null argument");
+  *n = (int)e->members.size();
+  return PM_OK;
+}
+
+PM_API int pm_ensemble_run(pm_ensemble* e, double* a, const double* da, int n_steps) {
+  if (!e) return fail(PM_E_INVALID, "null ensemble");
+  if (n_steps < 0) return fail(PM_E_INVALID, "n_steps must be >= 0");
+  const int nm = (int)e->members.size();
+  if (nm == 0 || n_steps == 0) return PM_OK;
+  if (!a || !da) return fail(PM_E_INVALID, "null a / da");
+  for (int i = 0; i < nm; ++i) {
+    pm_sim* s = e->members[i];
+    if (!s->has_state) return fail(PM_E_STATE, "ensemble member %d has no state", i);
+    if (int rc = ens_check_member(s)) return rc;
+    // a member's own stream must be idle, and its density and bins valid, before the batch
+    if (!s->rho_valid || !s->bins_valid)
+      if (int rc = sim_deposit_impl<float>(s)) return rc;
+    PM_HIP(hipStreamSynchronize(s->stream));
+  }
+  if (e->shape_dirty) {
+    ens_drop_graph(e);
+    e->shape_dirty = false;
+  }
+  // the descriptor is rewritten every call (member masses or Green's grids may have changed);
+  // the launch shapes only change with the membership
+  ens_describe(e);
+  PM_HIP(hipMemcpyAsync(e->d_desc, e->host, sizeof(BatchDesc), hipMemcpyHostToDevice, e->stream));
+  const bool use_graph = !e->timing && n_steps >= 2;
+  if (use_graph)
+    if (int rc = ens_build_graph(e)) return rc;
+  std::vector<double> acur(a, a + nm);
+  int done = 0;
+  while (done < n_steps) {
+    const int chunk = std::min(n_steps - done, kEnsTableSteps);
+    for (int k = 0; k < chunk; ++k)
+      for (int slot = 0; slot < nm; ++slot) {
+        const int i = e->order[slot];
+        if (!(acur[i] > 0.0) || !(acur[i] + da[i] > 0.0))
+          return fail(PM_E_INVALID, "scale factor of member %d must stay positive", i);
+        const StepScalars c = step_scalars(e->members[i], acur[i], da[i]);
+        double* t = &e->table[3 * ((size_t)k * nm + slot)];
+        t[0] = c.green;
+        t[1] = c.kick;
+        t[2] = c.drift;
+        acur[i] += da[i];  // pmesh.py:63
+      }
+    const int zero = 0;
+    PM_HIP(hipMemcpyAsync(e->d_table, e->table.data(), sizeof(double) * 3 * (size_t)chunk * nm,
+                          hipMemcpyHostToDevice, e->stream));
+    PM_HIP(hipMemcpyAsync(e->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    int k = 0;
+    if (use_graph)
+      for (; k + e->graph_steps <= chunk; k += e->graph_steps) PM_HIP(hipGraphLaunch(e->graph, e->stream));
+    for (; k < chunk; ++k)
+      if (int rc = ens_step(e)) return rc;
+    // the host table must outlive the asynchronous copies above
+    PM_HIP(hipStreamSynchronize(e->stream));
+    done += chunk;
+  }
+  for (int i = 0; i < nm; ++i) {
+    a[i] = acur[i];
+    pm_sim* s = e->members[i];
+    s->rho_valid = s->bins_valid = s->phi_valid = true;
+  }
+  return PM_OK;
+}
+
+PM_API int pm_ensemble_set_timing(pm_ensemble* e, int enable) {
+  if (!e) return fail(PM_E_INVALID, "null ensemble");
+  if (int rc = collect_timing(e)) return rc;
+  e->timing = enable != 0;
+  for (int i = 0; i < ST_COUNT; ++i) {
+    e->stage_ms[i] = 0;
+    e->stage_calls[i] = 0;
+  }
+  return PM_OK;
+}
+
+PM_API int pm_ensemble_get_timing(pm_ensemble* e, double* ms, int64_t* calls, int n_stages) {
+  if (!e || !ms || !calls) return fail(PM_E_INVALID, "null argument");
+  if (int rc = collect_timing(e)) return rc;
+  for (int i = 0; i < n_stages && i < ST_COUNT; ++i) {
+    ms[i] = e->stage_ms[i];
+    calls[i] = e->stage_calls[i];
+  }
+  return PM_OK;
+}
+
+PM_API int pm_ensemble_sync(pm_ensemble* e) {
+  if (!e) return fail(PM_E_INVALID, "null ensemble");
+  PM_HIP(hipStreamSynchronize(e->stream));
   return PM_OK;
 }
 

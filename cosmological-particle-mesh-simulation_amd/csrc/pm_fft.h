@@ -156,6 +156,19 @@ struct GreenArgs {
   const int* step = nullptr;
 };
 
+// A set of pencils of one pass: `nblk` blocks of W columns.  Contiguous segments hold W
+// adjacent columns per block (outer index, column block); strided ones hold W columns with the
+// FFT axis contiguous.
+struct PencilSeg {
+  long nblk;      // blocks in this segment
+  long ncol_blk;  // column blocks per outer index (contiguous segments only)
+  long outer_stride;
+  long stride_n;
+  long stride_w;  // strided segments: distance between columns
+  int contig;     // 1: W columns contiguous; 0: the FFT axis is contiguous
+  int green_kind; // 0: main H z-pass (kx = col, ky = outer); 1: Q z-pass (kx = N/2, ky = col)
+};
+
 // per-step scalars of a captured step: {green coefficient, kick, drift} per step
 struct StepTable {
   const double* table = nullptr;

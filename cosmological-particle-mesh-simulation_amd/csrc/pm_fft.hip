@@ -8,7 +8,9 @@
 //   c2r_x   : H rows + Q -> real rows, written over H in place
 // The real-to-complex step packs each row as N/2 complex samples z[n] = x[2n] + i x[2n+1],
 // transforms them with an N/2-point FFT and untangles the even/odd spectra.
-#include "pm_fft.h"
+#include <algorithm>
+
+#include "pm_batch.h"
 
 namespace pm {
 
@@ -38,18 +40,19 @@ template <typename R, int N> constexpr int rows_per_block() {
 // ------------------------------------------------------------------------------------------
 // x pass, real -> complex
 // ------------------------------------------------------------------------------------------
+// LDS of the x passes: M*(RB+1) complex for the rows plus N twiddles
+template <typename R, int N, int RB> constexpr int xpass_lds() { return ((N / 2) * (RB + 1) + N) * (int)sizeof(cpx<R>); }
+
 template <typename R, typename Tin, int N, int RB>
-__global__ __launch_bounds__(kThreads) void k_r2c_x(const Tin* __restrict__ rho, cpx<R>* __restrict__ H,
-                                                    cpx<R>* __restrict__ Q, const cpx<R>* __restrict__ twg) {
+__device__ __forceinline__ void r2c_body(const Tin* __restrict__ rho, cpx<R>* __restrict__ H, cpx<R>* __restrict__ Q,
+                                         const cpx<R>* __restrict__ twg, long blk, cpx<R>* lds, cpx<R>* tw) {
   using T = cpx<R>;
   constexpr int M = N / 2;
   constexpr int SN = RB + 1;
   constexpr int NT = kThreads;
-  __shared__ T lds[M * SN];
-  __shared__ T tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
-  const size_t row0 = (size_t)blockIdx.x * RB;
+  const size_t row0 = (size_t)blk * RB;
   const cpx<Tin>* src = reinterpret_cast<const cpx<Tin>*>(rho) + row0 * M;
   // the block's loads are issued in chunks of CH per thread before they are stored
   constexpr int LPT = (RB * M + NT - 1) / NT;
@@ -116,22 +119,28 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const Tin* __restrict__ rho,
   }
 }
 
+template <typename R, typename Tin, int N, int RB>
+__global__ __launch_bounds__(kThreads) void k_r2c_x(const Tin* __restrict__ rho, cpx<R>* __restrict__ H,
+                                                    cpx<R>* __restrict__ Q, const cpx<R>* __restrict__ twg) {
+  __shared__ cpx<R> lds[(N / 2) * (RB + 1)];
+  __shared__ cpx<R> tw[N];
+  r2c_body<R, Tin, N, RB>(rho, H, Q, twg, blockIdx.x, lds, tw);
+}
+
 // ------------------------------------------------------------------------------------------
 // x pass, complex -> real (in place allowed: H and out may be the same memory)
 // ------------------------------------------------------------------------------------------
 template <typename R, typename Tout, int N, int RB>
-__global__ __launch_bounds__(kThreads) void k_c2r_x(const cpx<R>* H, const cpx<R>* __restrict__ Q, Tout* out,
-                                                    const cpx<R>* __restrict__ twg) {
+__device__ __forceinline__ void c2r_body(const cpx<R>* H, const cpx<R>* __restrict__ Q, Tout* out,
+                                         const cpx<R>* __restrict__ twg, long blk, cpx<R>* lds, cpx<R>* tw,
+                                         cpx<R>* qx) {
   using T = cpx<R>;
   constexpr int M = N / 2;
   constexpr int SN = RB + 1;
   constexpr int NT = kThreads;
-  __shared__ T lds[M * SN];
-  __shared__ T tw[N];
-  __shared__ T qx[RB];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
-  const size_t row0 = (size_t)blockIdx.x * RB;
+  const size_t row0 = (size_t)blk * RB;
   const T* src = H + row0 * M;
   constexpr int LPT = (RB * M + NT - 1) / NT;
   constexpr int CH = sizeof(R) == 4 ? 32 : 8;
@@ -198,18 +207,18 @@ __global__ __launch_bounds__(kThreads) void k_c2r_x(const cpx<R>* H, const cpx<R
   }
 }
 
+template <typename R, typename Tout, int N, int RB>
+__global__ __launch_bounds__(kThreads) void k_c2r_x(const cpx<R>* H, const cpx<R>* __restrict__ Q, Tout* out,
+                                                    const cpx<R>* __restrict__ twg) {
+  __shared__ cpx<R> lds[(N / 2) * (RB + 1)];
+  __shared__ cpx<R> tw[N];
+  __shared__ cpx<R> qx[RB];
+  c2r_body<R, Tout, N, RB>(H, Q, out, twg, blockIdx.x, lds, tw, qx);
+}
+
 // ------------------------------------------------------------------------------------------
 // pencil passes along y or z
 // ------------------------------------------------------------------------------------------
-struct PencilSeg {
-  long nblk;      // blocks in this segment
-  long ncol_blk;  // column blocks per outer index (contiguous segments only)
-  long outer_stride;
-  long stride_n;
-  long stride_w;  // strided segments: distance between columns
-  int contig;     // 1: W columns contiguous; 0: the FFT axis is contiguous
-  int green_kind; // 0: main H z-pass (kx = col, ky = outer); 1: Q z-pass (kx = N/2, ky = col)
-};
 
 template <typename R>
 __device__ __forceinline__ R green_factor(int kx, int ky, int kz, int n, const GreenArgs& g, R coef, const R* s2,
@@ -226,20 +235,22 @@ __device__ __forceinline__ R green_factor(int kx, int ky, int kz, int n, const G
   return f;
 }
 
+// LDS of a pencil pass: L*(W+1) complex + N twiddles (+ 2N reals of k-space tables)
+template <typename R, int N> constexpr int pencil_lds() {
+  return (N * (pencil_w<R, N>() + 1) + N) * (int)sizeof(cpx<R>) + 2 * N * (int)sizeof(R);
+}
+
 // MODE: -1 forward only, +1 inverse only, 0 forward + k-space operator + inverse
 template <typename R, int N, int MODE>
-__global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, PencilSeg s0, PencilSeg s1, GreenArgs g,
-                                                     const cpx<R>* __restrict__ twg, const R* __restrict__ s2g,
-                                                     const R* __restrict__ wdg) {
+__device__ __forceinline__ void pencil_body(cpx<R>* H, cpx<R>* Q, const PencilSeg& s0, const PencilSeg& s1,
+                                            const GreenArgs& g, const cpx<R>* __restrict__ twg,
+                                            const R* __restrict__ s2g, const R* __restrict__ wdg, long blk,
+                                            cpx<R>* lds, cpx<R>* tw, R* s2, R* wd) {
   using T = cpx<R>;
   constexpr int L = N;
   constexpr int W = pencil_w<R, N>();
   constexpr int SN = W + 1;  // room for either layout
   constexpr int NT = kThreads;
-  __shared__ T lds[L * SN];
-  __shared__ T tw[N];
-  __shared__ R s2[MODE == 0 ? N : 1];
-  __shared__ R wd[MODE == 0 ? N : 1];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += NT) {
     tw[i] = twg[i];
@@ -248,7 +259,6 @@ __global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, Penci
       wd[i] = wdg[i];
     }
   }
-  long blk = blockIdx.x;
   const bool first = blk < s0.nblk;
   const PencilSeg& s = first ? s0 : s1;
   if (!first) blk -= s0.nblk;
@@ -332,6 +342,17 @@ __global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, Penci
       base[n + w * s.stride_w] = lds[n * SN + w];
     }
   }
+}
+
+template <typename R, int N, int MODE>
+__global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, PencilSeg s0, PencilSeg s1, GreenArgs g,
+                                                     const cpx<R>* __restrict__ twg, const R* __restrict__ s2g,
+                                                     const R* __restrict__ wdg) {
+  __shared__ cpx<R> lds[N * (pencil_w<R, N>() + 1)];
+  __shared__ cpx<R> tw[N];
+  __shared__ R s2[MODE == 0 ? N : 1];
+  __shared__ R wd[MODE == 0 ? N : 1];
+  pencil_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, blockIdx.x, lds, tw, s2, wd);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -481,5 +502,178 @@ template hipError_t fft_c2r_x<double, float>(cpx<double>*, const cpx<double>*, f
 PM_INST(float)
 PM_INST(double)
 #undef PM_INST
+
+// ------------------------------------------------------------------------------------------
+// batched passes: the blocks of every member of an ensemble in one launch (FP32)
+// ------------------------------------------------------------------------------------------
+namespace {
+
+template <int N> constexpr int x_rb() { return rows_per_block<float, N>(); }
+
+// the cases of size class CLS (the kernel's registers are the maximum over its cases, so the
+// large grids get a kernel of their own)
+#define PM_BATCH_CLASS_SWITCH(CLS, NV, CASE)                \
+  if constexpr (CLS == 0) {                                 \
+    switch (NV) { PM_FOR_EACH_BATCH_N_SMALL(CASE) default: break; } \
+  } else {                                                  \
+    switch (NV) { PM_FOR_EACH_BATCH_N_LARGE(CASE) default: break; } \
+  }
+
+template <int CLS>
+__global__ __launch_bounds__(kThreads) void k_batch_r2c(const BatchDesc* __restrict__ D, long base) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const long b = base + blockIdx.x;
+  const int k = batch_member(D, BS_R2C, b);
+  const BatchMember& m = D->m[k];
+  const long blk = b - D->off[BS_R2C][k];
+  cpx<float>* lds = reinterpret_cast<cpx<float>*>(smem);
+#define PM_CASE(NN)                                                                              \
+  case NN:                                                                                       \
+    r2c_body<float, float, NN, x_rb<NN>()>(m.rho, m.H, m.Q, m.tw, blk, lds,                      \
+                                           lds + (NN / 2) * (x_rb<NN>() + 1));                   \
+    break;
+  PM_BATCH_CLASS_SWITCH(CLS, m.n, PM_CASE)
+#undef PM_CASE
+}
+
+template <int CLS>
+__global__ __launch_bounds__(kThreads) void k_batch_c2r(const BatchDesc* __restrict__ D, long base) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const long b = base + blockIdx.x;
+  const int k = batch_member(D, BS_C2R, b);
+  const BatchMember& m = D->m[k];
+  const long blk = b - D->off[BS_C2R][k];
+  cpx<float>* lds = reinterpret_cast<cpx<float>*>(smem);
+#define PM_CASE(NN)                                                                              \
+  case NN: {                                                                                     \
+    cpx<float>* tw = lds + (NN / 2) * (x_rb<NN>() + 1);                                          \
+    c2r_body<float, float, NN, x_rb<NN>()>(m.H, m.Q, reinterpret_cast<float*>(m.H), m.tw, blk,   \
+                                           lds, tw, tw + NN);                                    \
+    break;                                                                                       \
+  }
+  PM_BATCH_CLASS_SWITCH(CLS, m.n, PM_CASE)
+#undef PM_CASE
+}
+
+template <int CLS, int MODE>
+__global__ __launch_bounds__(kThreads) void k_batch_pencil(const BatchDesc* __restrict__ D, long base) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int ST = MODE == 0 ? BS_Z : BS_Y;
+  const long b = base + blockIdx.x;
+  const int k = batch_member(D, ST, b);
+  const BatchMember& m = D->m[k];
+  const long blk = b - D->off[ST][k];
+  GreenArgs g{};
+  if (MODE == 0) {
+    g.coef = D->table[3 * ((long)(*D->step) * D->n_members + k)];
+    g.deconvolve = m.deconvolve;
+    g.from_grid = m.from_grid;
+    g.fgrid = m.fgrid;
+  }
+  cpx<float>* lds = reinterpret_cast<cpx<float>*>(smem);
+#define PM_CASE(NN)                                                                              \
+  case NN: {                                                                                     \
+    cpx<float>* tw = lds + NN * (pencil_w<float, NN>() + 1);                                     \
+    float* s2 = reinterpret_cast<float*>(tw + NN);                                               \
+    pencil_body<float, NN, MODE>(m.H, m.Q, MODE == 0 ? m.za : m.ya, MODE == 0 ? m.zb : m.yb, g,  \
+                                 m.tw, m.sin2, m.wdec, blk, lds, tw, s2, s2 + NN);               \
+    break;                                                                                       \
+  }
+  PM_BATCH_CLASS_SWITCH(CLS, m.n, PM_CASE)
+#undef PM_CASE
+}
+
+inline hipError_t launch_dyn(const void* fn, long base, long count, int lds, const BatchDesc* d, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  if (count > 0x7fffffffL) return hipErrorInvalidValue;
+  void* args[] = {(void*)&d, (void*)&base};
+  return hipLaunchKernel(fn, dim3((unsigned)count), dim3(kThreads), args, (size_t)lds, st);
+}
+
+}  // namespace
+
+bool batch_size_supported(int n) {
+  switch (n) {
+#define PM_CASE(NN) case NN:
+    PM_FOR_EACH_BATCH_N(PM_CASE)
+#undef PM_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+void batch_fft_segments(int n, PencilSeg& ya, PencilSeg& yb, PencilSeg& za, PencilSeg& zb) {
+  switch (n) {
+#define PM_CASE(NN)                    \
+  case NN:                             \
+    y_segments<float, NN>(ya, yb);     \
+    z_segments<float, NN>(za, zb);     \
+    return;
+    PM_FOR_EACH_BATCH_N(PM_CASE)
+#undef PM_CASE
+  }
+  ya = yb = za = zb = PencilSeg{};
+}
+
+long batch_fft_blocks(int n, int stage) {
+  switch (n) {
+#define PM_CASE(NN)                                                          \
+  case NN: {                                                                 \
+    PencilSeg a, b;                                                          \
+    if (stage == BS_R2C || stage == BS_C2R) return (long)NN * NN / x_rb<NN>(); \
+    if (stage == BS_Y) y_segments<float, NN>(a, b);                          \
+    else z_segments<float, NN>(a, b);                                        \
+    return a.nblk + b.nblk;                                                  \
+  }
+    PM_FOR_EACH_BATCH_N(PM_CASE)
+#undef PM_CASE
+  }
+  return 0;
+}
+
+int batch_fft_lds(int n, int stage) {
+  switch (n) {
+#define PM_CASE(NN)                                                                                  \
+  case NN:                                                                                           \
+    if (stage == BS_R2C) return xpass_lds<float, NN, x_rb<NN>()>();                                  \
+    if (stage == BS_C2R) return xpass_lds<float, NN, x_rb<NN>()>() + x_rb<NN>() * 8;                 \
+    return pencil_lds<float, NN>();
+    PM_FOR_EACH_BATCH_N(PM_CASE)
+#undef PM_CASE
+  }
+  return 0;
+}
+
+int batch_class(int n) { return n <= 256 ? 0 : 1; }
+
+hipError_t batch_fft_init() {
+  // blocks above 64 KiB of LDS are legal on gfx950: allow the batched FFT kernels the largest
+  // request any member size makes (set once, outside any stream capture)
+  int lds = 0;
+  for (int st = BS_R2C; st <= BS_C2R; ++st) lds = std::max(lds, batch_fft_lds(512, st));
+  const void* fns[] = {(const void*)k_batch_r2c<0>,         (const void*)k_batch_r2c<1>,
+                       (const void*)k_batch_c2r<0>,         (const void*)k_batch_c2r<1>,
+                       (const void*)k_batch_pencil<0, -1>, (const void*)k_batch_pencil<1, -1>,
+                       (const void*)k_batch_pencil<0, 0>,  (const void*)k_batch_pencil<1, 0>,
+                       (const void*)k_batch_pencil<0, 1>,  (const void*)k_batch_pencil<1, 1>};
+  for (const void* f : fns)
+    if (hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds)) return e;
+  return hipSuccess;
+}
+
+hipError_t batch_r2c(const BatchDesc* d, int cls, long base, long count, int lds, hipStream_t st) {
+  return launch_dyn(cls ? (const void*)k_batch_r2c<1> : (const void*)k_batch_r2c<0>, base, count, lds, d, st);
+}
+hipError_t batch_c2r(const BatchDesc* d, int cls, long base, long count, int lds, hipStream_t st) {
+  return launch_dyn(cls ? (const void*)k_batch_c2r<1> : (const void*)k_batch_c2r<0>, base, count, lds, d, st);
+}
+hipError_t batch_pencil(const BatchDesc* d, int cls, int mode, long base, long count, int lds, hipStream_t st) {
+  const void* fn;
+  if (mode < 0) fn = cls ? (const void*)k_batch_pencil<1, -1> : (const void*)k_batch_pencil<0, -1>;
+  else if (mode > 0) fn = cls ? (const void*)k_batch_pencil<1, 1> : (const void*)k_batch_pencil<0, 1>;
+  else fn = cls ? (const void*)k_batch_pencil<1, 0> : (const void*)k_batch_pencil<0, 0>;
+  return launch_dyn(fn, base, count, lds, d, st);
+}
 
 }  // namespace pm

@@ -14,7 +14,7 @@
 // The gather reads the own entries with the tile's potential (plus a -1/+2 stencil margin)
 // staged in LDS.  Counts come from a wave-aggregated atomic per distinct key (one atomic per
 // distinct tile in a wavefront), and the offsets from a three-kernel exclusive scan.
-#include "pm_particles.h"
+#include "pm_batch.h"
 
 namespace pm {
 
@@ -138,10 +138,10 @@ __global__ __launch_bounds__(kT) void k_bin_count(const R* __restrict__ s, Latti
 }
 
 template <typename R, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, Lattice L, TileGeom G,
-                                                    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursors,
-                                                    uint32_t* __restrict__ entries, uint32_t capacity) {
-  const long slot = blockIdx.x * (long)kT + threadIdx.x;
+__device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const Lattice& L, const TileGeom& G,
+                                                 const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursors,
+                                                 uint32_t* __restrict__ entries, uint32_t capacity, long blk) {
+  const long slot = blk * (long)kT + threadIdx.x;
   const bool valid = slot < L.np;
   int c[3] = {0, 0, 0}, t[3] = {0, 0, 0}, up[3] = {0, 0, 0};
   R d[3];
@@ -159,6 +159,13 @@ __global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, Lat
       if (pos < offsets[key + 1] && pos < capacity) entries[pos] = (uint32_t)slot;
     }
   }
+}
+
+template <typename R, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, Lattice L, TileGeom G,
+                                                    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursors,
+                                                    uint32_t* __restrict__ entries, uint32_t capacity) {
+  bin_scatter_body<R, TX, TY, TZ>(s, L, G, offsets, cursors, entries, capacity, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -183,10 +190,9 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   return incl - v;
 }
 
-__global__ __launch_bounds__(kT) void k_scan_reduce(const uint32_t* __restrict__ counts, long m,
-                                                    uint32_t* __restrict__ bsum) {
-  __shared__ uint32_t sh[kT];
-  const long base = blockIdx.x * (long)kScanChunk;
+__device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ counts, long m,
+                                                 uint32_t* __restrict__ bsum, long blk, uint32_t* sh) {
+  const long base = blk * (long)kScanChunk;
   uint32_t v = 0;
   for (int i = 0; i < kScanPer; ++i) {
     const long j = base + (long)i * kT + threadIdx.x;
@@ -194,11 +200,16 @@ __global__ __launch_bounds__(kT) void k_scan_reduce(const uint32_t* __restrict__
   }
   uint32_t total;
   block_exclusive_scan(v, sh, total);
-  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+  if (threadIdx.x == 0) bsum[blk] = total;
 }
 
-__global__ __launch_bounds__(kT) void k_scan_spine(uint32_t* __restrict__ bsum, int nb) {
+__global__ __launch_bounds__(kT) void k_scan_reduce(const uint32_t* __restrict__ counts, long m,
+                                                    uint32_t* __restrict__ bsum) {
   __shared__ uint32_t sh[kT];
+  scan_reduce_body(counts, m, bsum, blockIdx.x, sh);
+}
+
+__device__ __forceinline__ void scan_spine_body(uint32_t* __restrict__ bsum, int nb, uint32_t* sh) {
   uint32_t carry = 0;
   for (int base = 0; base < nb; base += kT) {
     const int j = base + threadIdx.x;
@@ -210,11 +221,15 @@ __global__ __launch_bounds__(kT) void k_scan_spine(uint32_t* __restrict__ bsum, 
   }
 }
 
-__global__ __launch_bounds__(kT) void k_scan_apply(uint32_t* __restrict__ counts, long m,
-                                                   const uint32_t* __restrict__ bsum, uint32_t* __restrict__ offsets,
-                                                   uint32_t* __restrict__ cursors) {
+__global__ __launch_bounds__(kT) void k_scan_spine(uint32_t* __restrict__ bsum, int nb) {
   __shared__ uint32_t sh[kT];
-  const long base = blockIdx.x * (long)kScanChunk + (long)threadIdx.x * kScanPer;
+  scan_spine_body(bsum, nb, sh);
+}
+
+__device__ __forceinline__ void scan_apply_body(uint32_t* __restrict__ counts, long m,
+                                                const uint32_t* __restrict__ bsum, uint32_t* __restrict__ offsets,
+                                                uint32_t* __restrict__ cursors, long blk, uint32_t* sh) {
+  const long base = blk * (long)kScanChunk + (long)threadIdx.x * kScanPer;
   uint32_t v[kScanPer];
   uint32_t sum = 0;
 #pragma unroll
@@ -224,7 +239,7 @@ __global__ __launch_bounds__(kT) void k_scan_apply(uint32_t* __restrict__ counts
     sum += v[i];
   }
   uint32_t total;
-  uint32_t run = bsum[blockIdx.x] + block_exclusive_scan(sum, sh, total);
+  uint32_t run = bsum[blk] + block_exclusive_scan(sum, sh, total);
 #pragma unroll
   for (int i = 0; i < kScanPer; ++i) {
     const long j = base + i;
@@ -238,6 +253,13 @@ __global__ __launch_bounds__(kT) void k_scan_apply(uint32_t* __restrict__ counts
   }
 }
 
+__global__ __launch_bounds__(kT) void k_scan_apply(uint32_t* __restrict__ counts, long m,
+                                                   const uint32_t* __restrict__ bsum, uint32_t* __restrict__ offsets,
+                                                   uint32_t* __restrict__ cursors) {
+  __shared__ uint32_t sh[kT];
+  scan_apply_body(counts, m, bsum, offsets, cursors, blockIdx.x, sh);
+}
+
 // ------------------------------------------------------------------------------------------
 // owner-computes CIC deposit (density.py:17-47 for the cells of one tile)
 //
@@ -246,18 +268,19 @@ __global__ __launch_bounds__(kT) void k_scan_apply(uint32_t* __restrict__ counts
 // ------------------------------------------------------------------------------------------
 constexpr uint32_t kNone = 0xffffffffu;
 
+// row pitch TX + 3: consecutive lattice particles (2 cells apart) then spread over the LDS
+// banks instead of sharing the few that a power-of-two pitch leaves them
+template <int TX, int TY, int TZ> constexpr int deposit_lds_cells() { return (TX + 3) * TY * TZ; }
+
 template <typename R, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R* __restrict__ rho, R mass,
-                                                      Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
-                                                      const uint32_t* __restrict__ entries, uint32_t capacity) {
+__device__ __forceinline__ void deposit_body(const R* __restrict__ s, R* __restrict__ rho, R mass, const Lattice& L,
+                                             const TileGeom& G, const uint32_t* __restrict__ offsets,
+                                             const uint32_t* __restrict__ entries, uint32_t capacity, int tile,
+                                             double* acc) {
   constexpr int CELLS = TX * TY * TZ;
   static_assert(CELLS % kT == 0, "tile must hold a multiple of the block size");
   constexpr int E = 4;
-  // row pitch TX + 3: consecutive lattice particles (2 cells apart) then spread over the LDS
-  // banks instead of sharing the few that a power-of-two pitch leaves them
-  constexpr int PX = TX + 3, PLANE = PX * TY, PADDED = PLANE * TZ;
-  __shared__ double acc[PADDED];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  constexpr int PX = TX + 3, PLANE = PX * TY, PADDED = deposit_lds_cells<TX, TY, TZ>();
   const int tix = tile % G.ntx, tiy = (tile / G.ntx) % G.nty, tiz = tile / (G.ntx * G.nty);
   const int o0[3] = {tix * TX, tiy * TY, tiz * TZ};
   const int T[3] = {TX, TY, TZ};
@@ -333,6 +356,15 @@ __global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R
   }
 }
 
+template <typename R, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R* __restrict__ rho, R mass,
+                                                      Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
+                                                      const uint32_t* __restrict__ entries, uint32_t capacity) {
+  __shared__ double acc[deposit_lds_cells<TX, TY, TZ>()];
+  deposit_body<R, TX, TY, TZ>(s, rho, mass, L, G, offsets, entries, capacity, xcd_tile(blockIdx.x, gridDim.x),
+                              acc);
+}
+
 // ------------------------------------------------------------------------------------------
 // tile-staged gather + kick + drift (integrate.py:15-97), counting the new tiles
 //
@@ -344,22 +376,22 @@ __global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R
 // the new position are counted for the next binning.  The potential box is fetched in
 // batches of CH loads per thread before they are written to LDS.
 // ------------------------------------------------------------------------------------------
+template <int TX, int TY, int TZ> constexpr int gkd_box_cells() { return (TX + 3) * (TY + 3) * (TZ + 3); }
+
 template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
-__global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi,
-                                                  R kick, R drift, Lattice L, TileGeom G,
-                                                  const uint32_t* __restrict__ offsets,
-                                                  const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
-                                                  uint32_t capacity, StepTable tab) {
-  constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3, BN = BX * BY * BZ;
+__device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi, R kick,
+                                         R drift, const Lattice& L, const TileGeom& G,
+                                         const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ entries,
+                                         uint32_t* __restrict__ counts, uint32_t capacity, int tile, R* box,
+                                         uint32_t* hist) {
+  constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3;
   static_assert(BX <= 64, "a box row must fit in one wavefront");
   constexpr int RPW = 64 / BX;                      // box rows per wavefront load
   constexpr int NW = kT / 64;                       // wavefronts per block
   constexpr int ROWS = BY * BZ;
   constexpr int ITERS = (ROWS + RPW * NW - 1) / (RPW * NW);
   constexpr int E = 2;
-  __shared__ R box[BN];
-  __shared__ uint32_t hist[128];  // new-tile counts: offset (-1..2)^3 from this tile x {own, halo}
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  // hist: new-tile counts, offset (-1..2)^3 from this tile x {own, halo}
   const uint32_t end = min(offsets[2 * tile + 1], capacity);
   const uint32_t beg = min(offsets[2 * tile], end);
   if (beg == end) return;  // empty tile: the whole block leaves together
@@ -418,11 +450,6 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
   }
   __syncthreads();
   const R nr = R(n);
-  if (tab.table) {  // graph replay: this step's kick and drift from the device table
-    const int k = *tab.step;
-    kick = R(tab.table[3 * k + 1]);
-    drift = R(tab.table[3 * k + 2]);
-  }
   for (uint32_t base = beg; base < end; base += kT * E) {
     if (base != beg) {
       load_entries(base);
@@ -528,6 +555,86 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
   }
 }
 
+template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
+__global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi,
+                                                  R kick, R drift, Lattice L, TileGeom G,
+                                                  const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
+                                                  uint32_t capacity, StepTable tab) {
+  __shared__ R box[gkd_box_cells<TX, TY, TZ>()];
+  __shared__ uint32_t hist[128];
+  if (tab.table) {  // graph replay: this step's kick and drift from the device table
+    const int k = *tab.step;
+    kick = R(tab.table[3 * k + 1]);
+    drift = R(tab.table[3 * k + 2]);
+  }
+  gkd_body<R, TX, TY, TZ, LOCAL_HIST>(s, v, phi, kick, drift, L, G, offsets, entries, counts, capacity,
+                                      xcd_tile(blockIdx.x, gridDim.x), box, hist);
+}
+
+// ------------------------------------------------------------------------------------------
+// batched tile kernels: the tiles (or particle blocks) of every member in one launch, FP32,
+// 16^3 tiles.  The XCD remap is applied to the ensemble-wide block index, so every XCD works
+// on one contiguous run of tiles.
+// ------------------------------------------------------------------------------------------
+constexpr int kBT = 16;
+
+__global__ __launch_bounds__(kT) void k_batch_gkd(const BatchDesc* __restrict__ D) {
+  __shared__ float box[gkd_box_cells<kBT, kBT, kBT>()];
+  __shared__ uint32_t hist[128];
+  const long b = xcd_tile(blockIdx.x, gridDim.x);
+  const int k = batch_member(D, BS_TILES, b);
+  const BatchMember& m = D->m[k];
+  const int tile = (int)(b - D->off[BS_TILES][k]);
+  const double* sc = D->table + 3 * ((long)(*D->step) * D->n_members + k);
+  const float kick = float(sc[1]), drift = float(sc[2]);
+  if (m.local_hist)
+    gkd_body<float, kBT, kBT, kBT, true>(m.s, m.v, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
+                                         m.offsets, m.entries, m.counts, m.capacity, tile, box, hist);
+  else
+    gkd_body<float, kBT, kBT, kBT, false>(m.s, m.v, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
+                                          m.offsets, m.entries, m.counts, m.capacity, tile, box, hist);
+}
+
+__global__ __launch_bounds__(kT) void k_batch_deposit(const BatchDesc* __restrict__ D) {
+  __shared__ double acc[deposit_lds_cells<kBT, kBT, kBT>()];
+  const long b = xcd_tile(blockIdx.x, gridDim.x);
+  const int k = batch_member(D, BS_TILES, b);
+  const BatchMember& m = D->m[k];
+  deposit_body<float, kBT, kBT, kBT>(m.s, m.rho, m.mass, m.L, m.G, m.offsets, m.entries, m.capacity,
+                                     (int)(b - D->off[BS_TILES][k]), acc);
+}
+
+__global__ __launch_bounds__(kT) void k_batch_scatter(const BatchDesc* __restrict__ D) {
+  const long b = blockIdx.x;
+  const int k = batch_member(D, BS_SCATTER, b);
+  const BatchMember& m = D->m[k];
+  bin_scatter_body<float, kBT, kBT, kBT>(m.s, m.L, m.G, m.offsets, m.cursors, m.entries, m.capacity,
+                                         b - D->off[BS_SCATTER][k]);
+}
+
+__global__ __launch_bounds__(kT) void k_batch_scan_reduce(const BatchDesc* __restrict__ D) {
+  __shared__ uint32_t sh[kT];
+  const long b = blockIdx.x;
+  const int k = batch_member(D, BS_SCAN, b);
+  const BatchMember& m = D->m[k];
+  scan_reduce_body(m.counts, 2L * m.G.ntiles, m.block_sums, b - D->off[BS_SCAN][k], sh);
+}
+
+__global__ __launch_bounds__(kT) void k_batch_scan_spine(const BatchDesc* __restrict__ D) {
+  __shared__ uint32_t sh[kT];
+  const BatchMember& m = D->m[blockIdx.x];
+  scan_spine_body(m.block_sums, m.scan_nb, sh);
+}
+
+__global__ __launch_bounds__(kT) void k_batch_scan_apply(const BatchDesc* __restrict__ D) {
+  __shared__ uint32_t sh[kT];
+  const long b = blockIdx.x;
+  const int k = batch_member(D, BS_SCAN, b);
+  const BatchMember& m = D->m[k];
+  scan_apply_body(m.counts, 2L * m.G.ntiles, m.block_sums, m.offsets, m.cursors, b - D->off[BS_SCAN][k], sh);
+}
+
 __global__ void k_step_advance(int* step) {
   if (threadIdx.x == 0) *step += 1;
 }
@@ -619,6 +726,45 @@ hipError_t gkd_tiles(R* s, R* v, const R* phi, double kick, double drift, const 
   } while (0)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
+  return hipGetLastError();
+}
+
+long batch_scatter_blocks(long np) { return blocks_for(np); }
+
+static hipError_t batch_grid(long total, unsigned& g) {
+  if (total > 0x7fffffffL) return hipErrorInvalidValue;
+  g = (unsigned)total;
+  return hipSuccess;
+}
+
+hipError_t batch_gkd(const BatchDesc* d, long total, hipStream_t st) {
+  unsigned g;
+  if (total <= 0) return hipSuccess;
+  if (hipError_t e = batch_grid(total, g)) return e;
+  k_batch_gkd<<<g, kT, 0, st>>>(d);
+  return hipGetLastError();
+}
+hipError_t batch_deposit(const BatchDesc* d, long total, hipStream_t st) {
+  unsigned g;
+  if (total <= 0) return hipSuccess;
+  if (hipError_t e = batch_grid(total, g)) return e;
+  k_batch_deposit<<<g, kT, 0, st>>>(d);
+  return hipGetLastError();
+}
+hipError_t batch_scatter(const BatchDesc* d, long total, hipStream_t st) {
+  unsigned g;
+  if (total <= 0) return hipSuccess;
+  if (hipError_t e = batch_grid(total, g)) return e;
+  k_batch_scatter<<<g, kT, 0, st>>>(d);
+  return hipGetLastError();
+}
+hipError_t batch_scan(const BatchDesc* d, long total, int n_members, hipStream_t st) {
+  unsigned g;
+  if (total <= 0 || n_members <= 0) return hipSuccess;
+  if (hipError_t e = batch_grid(total, g)) return e;
+  k_batch_scan_reduce<<<g, kT, 0, st>>>(d);
+  k_batch_scan_spine<<<n_members, kT, 0, st>>>(d);
+  k_batch_scan_apply<<<g, kT, 0, st>>>(d);
   return hipGetLastError();
 }
 

@@ -69,6 +69,15 @@ SIGNATURES = {
     "pm_sim_get_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_i64), _i]),
     "pm_sim_sync": (_i, [_vp]),
     "pm_sim_footprint": (_i, [_i, _i, _i, C.POINTER(_sz)]),
+    "pm_ensemble_create": (_i, [C.POINTER(_vp)]),
+    "pm_ensemble_destroy": (_i, [_vp]),
+    "pm_ensemble_add": (_i, [_vp, _vp]),
+    "pm_ensemble_remove": (_i, [_vp, _vp]),
+    "pm_ensemble_size": (_i, [_vp, C.POINTER(_i)]),
+    "pm_ensemble_run": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), _i]),
+    "pm_ensemble_set_timing": (_i, [_vp, _i]),
+    "pm_ensemble_get_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_i64), _i]),
+    "pm_ensemble_sync": (_i, [_vp]),
 }
 
 _lock = threading.Lock()

@@ -131,6 +131,30 @@ PM_API int pm_sim_sync(pm_sim* sim);
 /* bytes of device memory held by the sim                                                     */
 PM_API int pm_sim_footprint(int n_cells, int n_parts_side, int precision, size_t* bytes);
 
+/* ------------------------------------------- batched ensemble (many sims, one launch/stage) */
+/* No single reference function corresponds: the reference runs one realisation per process
+ * (pmesh.py:54-77).  An ensemble advances many FP32 sims, of grid sizes 32..512 (2^a 3^b,
+ * multiples of 16) that may differ, with one kernel launch per stage of the step for all of
+ * them (two for the FFT stages when grids above 256 are mixed with smaller ones); the steps
+ * replay a captured graph.  Members stay ordinary sims: they keep their buffers, can be stepped
+ * alone between ensemble calls and leave the ensemble when destroyed.                         */
+typedef struct pm_ensemble pm_ensemble;
+PM_API int pm_ensemble_create(pm_ensemble** out);
+PM_API int pm_ensemble_destroy(pm_ensemble* ens);   /* the member sims are left alone          */
+/* add a sim (precision 32, a supported size, not in another ensemble); at most 64 members   */
+PM_API int pm_ensemble_add(pm_ensemble* ens, pm_sim* sim);
+PM_API int pm_ensemble_remove(pm_ensemble* ens, pm_sim* sim);
+PM_API int pm_ensemble_size(pm_ensemble* ens, int* n_members);
+/* n_steps steps of every member: member i (in the order of adding) starts at a[i] with step
+ * da[i] (pmesh.py:56-63 per member); a[i] is advanced in place.                             */
+PM_API int pm_ensemble_run(pm_ensemble* ens, double* a, const double* da, int n_steps);
+/* per-stage timing of the batched launches (same stage order as pm_sim_get_timing); while on,
+ * steps are enqueued one by one instead of replaying the graph                               */
+PM_API int pm_ensemble_set_timing(pm_ensemble* ens, int enable);
+PM_API int pm_ensemble_get_timing(pm_ensemble* ens, double* stage_ms, int64_t* stage_calls,
+                                  int n_stages);
+PM_API int pm_ensemble_sync(pm_ensemble* ens);
+
 #ifdef __cplusplus
 }
 #endif
