@@ -63,12 +63,16 @@ def step_bytes(n, n_p, prec):
     return (60 * n_p + 32 * n ** 3 + 16 * n * n * (n // 2 + 1)) * (prec // 32)
 
 
+def _ic_params(w):
+    return dict(n_parts=w["n_parts"], n_cells=w["n_cells"], box=w["box"], seed=w["seed"], om=w["om"],
+                ob=w["ob"], ol=w["ol"], ok=w["ok"], h=w["h"], ns=w["ns"], sigma8=w["sigma8"],
+                a_init=w["a_init"])
+
+
 def make_ics(w):
     import contextlib
     import initial_conditions as ic
-    p = dict(n_parts=w["n_parts"], n_cells=w["n_cells"], box=w["box"], seed=w["seed"], om=w["om"],
-             ob=w["ob"], ol=w["ol"], ok=w["ok"], h=w["h"], ns=w["ns"], sigma8=w["sigma8"],
-             a_init=w["a_init"])
+    p = _ic_params(w)
     with contextlib.redirect_stdout(sys.stderr):   # stdout carries only the JSON line
         delta = ic.gaussian_random_field(p)
         return ic.zeldovich_numpy(delta, p)
@@ -106,13 +110,125 @@ def stage_timing(pos, vel, w, steps):
     return {k: ms / max(c, 1) for k, (ms, c) in t.items()}, {k: c for k, (ms, c) in t.items()}
 
 
-def pk_error(r32, r64, box):
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import pm_oracle as O
-    k, p32, _ = O.power_spectrum(r32, box)
-    _, p64, _ = O.power_spectrum(r64, box)
-    n = r32.shape[0]
-    sel = np.arange(1, len(k) + 1) <= n // 4          # k < k_Nyquist / 2
+def plain_fp64_bench(pos, vel, w, steps):
+    """The reference's algorithm as plain GPU code: the reference-layout entry points
+    (density(): global float atomics into a new grid; advance_time(): complex128 FFT Poisson
+    solve, float32 potential, reference integrate with pre-step cells for all axes), absolute
+    float32 positions, called once per step like pmesh.py:60-61."""
+    from pmgpu import ops
+    from pmgpu.device import DeviceArray
+    n, npart = w["n_cells"], w["n_parts"]
+    mass = (n / npart) ** 3
+    da = (w["a_end"] - w["a_init"]) / w["steps"]
+    x = DeviceArray.from_numpy(pos)
+    v = DeviceArray.from_numpy(vel)
+    g = ops.fourier_grid(n, host=False)
+    cf = (w["h"], w["ol"], w["ok"])        # integrate.py:12 passes [H0, OL, OK]
+    a = w["a_init"]
+    rho = ops.density(x, mass, n)          # warm-up: workspaces, tables
+    ops.advance_time(rho, x, v, g, a, da, cf, w["om"])
+    a += da
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rho = ops.density(x, mass, n)
+        ops.advance_time(rho, x, v, g, a, da, cf, w["om"])   # synchronous, like the reference
+        a += da
+    dt = time.perf_counter() - t0
+    return {"value": npart ** 3 * steps / dt, "ms_per_step": dt / steps * 1e3, "steps": steps,
+            "path": "reference-layout entry points pm_density + pm_advance_time (float32 "
+                    "storage, complex128 FFT, global atomics, one call per reference function)"}
+
+
+# Batched ensemble workload: 48 realisations, seeds 100-147, N_grid uniform over 128/192/256
+# with (N_grid/2)^3 particles, box U(50, 300) Mpc/h, sigma8 U(0.7, 0.9), 200 steps a 0.02 -> 1.
+ENSEMBLE = dict(members=48, seed0=100, sizes=(128, 192, 256), box=(50.0, 300.0),
+                sigma8=(0.7, 0.9), steps=200, a_init=0.02, a_end=1.0, meta_seed=2024)
+
+
+def loop_steps(a_init, a_end, steps):
+    """Steps the reference's host loop executes (pmesh.py:56-63, float64 accumulation of a)."""
+    da = (a_end - a_init) / steps
+    a, k = a_init, 0
+    while a < a_end - da:
+        a += da
+        k += 1
+    return k
+
+
+def ensemble_members(e):
+    """The ensemble's per-member draws from a fixed meta generator (documented seed)."""
+    rng = np.random.default_rng(e["meta_seed"])
+    out = []
+    for i in range(e["members"]):
+        n = int(rng.choice(e["sizes"]))
+        out.append(dict(WORKLOAD, n_cells=n, n_parts=n // 2, seed=e["seed0"] + i,
+                        box=float(rng.uniform(*e["box"])), sigma8=float(rng.uniform(*e["sigma8"]))))
+    return out
+
+
+def ensemble_bench(e, steps=None):
+    """All members in one ensemble (one launch per stage, graph-replayed) against the same
+    members run one at a time, each with its own graph."""
+    from pmgpu.ensemble import PMEnsemble
+    from pmgpu.sim import PMSim
+    members = ensemble_members(e)
+    n_steps = steps or loop_steps(e["a_init"], e["a_end"], e["steps"])
+    da = (e["a_end"] - e["a_init"]) / e["steps"]
+    import initial_conditions as ic
+    log(f"ensemble: {len(members)} members, initial conditions on the device (Philox seeds)")
+    sims = []
+    for m in members:
+        s = PMSim(m["n_cells"], m["n_parts"], 32, omega_m0=m["om"], omega_lambda0=m["ol"],
+                  omega_k0=m["ok"], h0=m["h"])
+        ic.zeldovich_device(s, _ic_params(m))
+        s.deposit()
+        sims.append(s)
+    # the members' starting state, restored before each timed run
+    ics = [s.get_state() for s in sims]
+    updates = sum(m["n_parts"] ** 3 for m in members) * n_steps
+    bytes_step = sum(step_bytes(m["n_cells"], m["n_parts"] ** 3, 32) for m in members)
+    ens = PMEnsemble(sims)
+    ens.run([e["a_init"]] * len(sims), [da] * len(sims), 8)        # warm-up: capture + replay
+    for s, (pos, vel) in zip(sims, ics):
+        s.set_state(pos, vel)
+        s.deposit()
+    log(f"ensemble: {n_steps} batched steps")
+    t0 = time.perf_counter()
+    ens.run([e["a_init"]] * len(sims), [da] * len(sims), n_steps)
+    ens.sync()
+    t_ens = time.perf_counter() - t0
+    ens.close()
+    log(f"ensemble: {t_ens:.3f} s; one at a time")
+    t_solo = 0.0
+    for s, (pos, vel) in zip(sims, ics):
+        s.run(e["a_init"], da, 8)                                   # capture + warm-up
+        s.set_state(pos, vel)
+        s.deposit()
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(e["a_init"], da, n_steps)
+        s.sync()
+        t_solo += time.perf_counter() - t0
+        s.close()
+    log(f"one at a time: {t_solo:.3f} s")
+    counts = {n: sum(1 for m in members if m["n_cells"] == n) for n in e["sizes"]}
+    return {"workload": f"ensemble_{len(members)}: seeds {e['seed0']}-{e['seed0'] + len(members) - 1}, "
+                        f"N_grid in {list(e['sizes'])} (counts {counts}), (N_grid/2)^3 particles, "
+                        f"box U{e['box']}, sigma8 U{e['sigma8']}, a {e['a_init']} -> {e['a_end']}",
+            "steps": n_steps, "particle_updates": updates,
+            "value": updates / t_ens, "unit": "particle-updates/s", "seconds": t_ens,
+            "ms_per_step": t_ens / n_steps * 1e3,
+            "roofline_frac": bytes_step * n_steps / t_ens / 1e9 / HBM_PEAK_GBS,
+            "one_at_a_time": {"value": updates / t_solo, "seconds": t_solo},
+            "speedup_vs_one_at_a_time": t_solo / t_ens}
+
+
+def pk_error(sim32, sim64, box):
+    """max |P32/P64 - 1| over k < k_Nyquist/2, both spectra from the device estimator."""
+    _, p32, _ = sim32.power_spectrum(box)
+    _, p64, _ = sim64.power_spectrum(box)
+    n = sim32.n_cells
+    sel = np.arange(1, len(p32) + 1) <= n // 4          # k < k_Nyquist / 2
     return float(np.max(np.abs(p32[sel] / p64[sel] - 1)))
 
 
@@ -171,6 +287,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--ensemble-steps", type=int, default=None)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a known-size kernel for "
                          "calibrating FETCH_SIZE / WRITE_SIZE counters)")
@@ -203,7 +322,7 @@ def main():
     log(f"FP64: {dt64:.3f} s")
     pk_err = None
     if fp64_steps == args.steps:
-        pk_err = pk_error(sim32.density(np.float64), sim64.density(np.float64), w["box"])
+        pk_err = pk_error(sim32, sim64, w["box"])
     sim32.close()
     sim64.close()
 
@@ -248,6 +367,10 @@ def main():
                      "avg_launch_ms": stage_ms[dom], "launches": stage_calls[dom]},
         "vs_reference_readme_cpu": v32 / 4.66e6,
     }
+    log("plain FP64 GPU path (reference-layout entry points)")
+    out["plain_fp64"] = plain_fp64_bench(pos, vel, w, args.plain_steps)
+    out["fp32_over_plain_fp64"] = v32 / out["plain_fp64"]["value"]
+    out["ensemble"] = None if args.no_ensemble else ensemble_bench(ENSEMBLE, args.ensemble_steps)
     if not args.no_cpu:
         log("CPU baseline (port) on host cores")
         out["cpu_baseline"] = cpu_baseline(pos, vel, w, args.cpu_steps, args.cpu_budget)

@@ -12,6 +12,7 @@
 #include "../../include/pm.h"
 #include "pm_batch.h"
 #include "pm_fft.h"
+#include "pm_ics.h"
 #include "pm_particles.h"
 
 using namespace pm;
@@ -937,6 +938,103 @@ PM_API int pm_sim_get_timing(pm_sim* s, double* ms, int64_t* calls, int n_stages
 PM_API int pm_sim_sync(pm_sim* s) {
   if (!s) return fail(PM_E_INVALID, "null sim");
   PM_HIP(hipStreamSynchronize(s->stream));
+  return PM_OK;
+}
+
+PM_API int pm_white_noise(double* out, int64_t n, uint64_t seed) {
+  PM_INIT();
+  if (n < 0 || (n && !out)) return fail(PM_E_INVALID, "pm_white_noise: bad arguments");
+  PM_HIP(white_noise(out, 8, n, seed, g_stream));
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+PM_API int pm_sim_zeldovich(pm_sim* s, const double* amp, int64_t n_amp, double psi_scale, double vel_factor,
+                            uint64_t seed, const void* noise, int noise_bytes) {
+  if (!s || !amp) return fail(PM_E_INVALID, "pm_sim_zeldovich: null argument");
+  const int n = s->L.np_side, N = s->L.n;
+  if (!fft_size_supported(n))
+    return fail(PM_E_UNSUPPORTED, "N_PARTS=%d is not a supported FFT size for the device generator", n);
+  // the noise spectrum lives in the density buffer, the components in the potential buffers
+  if ((double)n * n * n + 2.0 * n * n > (double)N * N * N || n > N)
+    return fail(PM_E_UNSUPPORTED, "the device generator needs N_PARTS < N_CELLS");
+  const int64_t need = 3 * (int64_t)(n / 2) * (n / 2) + 1;
+  if (n_amp < need) return fail(PM_E_INVALID, "h_amp needs %lld entries", (long long)need);
+  if (noise && noise_bytes != 4 && noise_bytes != 8) return fail(PM_E_INVALID, "noise_dtype_bytes must be 4 or 8");
+  FftTables* t;
+  if (int rc = get_tables(n, s->prec, &t)) return rc;
+  double* d_amp = nullptr;
+  if (hipMalloc(&d_amp, sizeof(double) * need) != hipSuccess) return fail(PM_E_NOMEM, "amplitude table allocation failed");
+  hipError_t e = hipMemcpyAsync(d_amp, amp, sizeof(double) * need, hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) {
+    if (s->prec == 4)
+      e = zeldovich_ics<float>(noise, noise_bytes, seed, d_amp, psi_scale, vel_factor, s->L, *t, (float*)s->s,
+                               (float*)s->v, (float*)s->rho, (float*)s->H, (cpx<float>*)s->Q, s->stream);
+    else
+      e = zeldovich_ics<double>(noise, noise_bytes, seed, d_amp, psi_scale, vel_factor, s->L, *t, (double*)s->s,
+                                (double*)s->v, (double*)s->rho, (double*)s->H, (cpx<double>*)s->Q, s->stream);
+  }
+  hipError_t es = hipStreamSynchronize(s->stream);
+  (void)hipFree(d_amp);
+  if (e != hipSuccess || es != hipSuccess)
+    return fail(PM_E_HIP, "device initial conditions failed: %s", hipGetErrorString(e ? e : es));
+  s->has_state = true;
+  s->rho_valid = s->bins_valid = s->phi_valid = false;
+  return PM_OK;
+}
+
+PM_API int pm_sim_power_spectrum(pm_sim* s, double box, int n_bins, double* k, double* pk, double* counts) {
+  if (!s || !k || !pk || !counts) return fail(PM_E_INVALID, "pm_sim_power_spectrum: null argument");
+  if (!s->has_state) return fail(PM_E_STATE, "state was never set");
+  if (n_bins < 1 || n_bins > 2048) return fail(PM_E_INVALID, "n_bins must be in 1..2048");
+  if (!(box > 0.0)) return fail(PM_E_INVALID, "box must be positive");
+  if (!s->rho_valid) {
+    int rc = s->prec == 4 ? sim_deposit_impl<float>(s) : sim_deposit_impl<double>(s);
+    if (rc) return rc;
+  }
+  const int n = s->L.n;
+  const long cells = (long)n * n * n;
+  double* d = nullptr;  // [0]: sum of rho, then 3 (n_bins + 1) shell sums
+  const size_t nd = 1 + 3 * (size_t)(n_bins + 1);
+  if (hipMalloc(&d, sizeof(double) * nd) != hipSuccess) return fail(PM_E_NOMEM, "spectrum scratch allocation failed");
+  std::vector<double> h(nd);
+  hipError_t e = hipMemsetAsync(d, 0, sizeof(double) * nd, s->stream);
+  if (e == hipSuccess) {
+    if (s->prec == 4) {
+      cpx<float>* H = (cpx<float>*)s->H;
+      cpx<float>* Q = (cpx<float>*)s->Q;
+      e = grid_sum<float>((const float*)s->rho, cells, d, s->stream);
+      if (!e) e = fft_r2c_x<float, float>((const float*)s->rho, H, Q, n, *s->tab, s->stream);
+      if (!e) e = fft_c2c_y<float>(H, Q, n, -1, *s->tab, s->stream);
+      if (!e) e = fft_c2c_z<float>(H, Q, n, -1, *s->tab, s->stream);
+      if (!e) e = shell_sums<float>(H, Q, n, n_bins, d + 1, s->stream);
+    } else {
+      cpx<double>* H = (cpx<double>*)s->H;
+      cpx<double>* Q = (cpx<double>*)s->Q;
+      e = grid_sum<double>((const double*)s->rho, cells, d, s->stream);
+      if (!e) e = fft_r2c_x<double, double>((const double*)s->rho, H, Q, n, *s->tab, s->stream);
+      if (!e) e = fft_c2c_y<double>(H, Q, n, -1, *s->tab, s->stream);
+      if (!e) e = fft_c2c_z<double>(H, Q, n, -1, *s->tab, s->stream);
+      if (!e) e = shell_sums<double>(H, Q, n, n_bins, d + 1, s->stream);
+    }
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, sizeof(double) * nd, hipMemcpyDeviceToHost, s->stream);
+  hipError_t es = hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  s->phi_valid = false;  // the potential buffer held the spectrum
+  if (e != hipSuccess || es != hipSuccess)
+    return fail(PM_E_HIP, "power spectrum failed: %s", hipGetErrorString(e ? e : es));
+  const double mean = h[0] / (double)cells;
+  if (!(mean > 0.0)) return fail(PM_E_STATE, "mean density is not positive");
+  const double norm = box * box * box / ((double)cells * (double)cells) / (mean * mean);
+  const double* psum = &h[1];
+  const double* ksum = psum + (n_bins + 1);
+  const double* cnt = ksum + (n_bins + 1);
+  for (int b = 1; b <= n_bins; ++b) {
+    counts[b - 1] = cnt[b];
+    k[b - 1] = cnt[b] > 0 ? ksum[b] / cnt[b] : 0.0;
+    pk[b - 1] = cnt[b] > 0 ? psum[b] / cnt[b] * norm : 0.0;
+  }
   return PM_OK;
 }
 

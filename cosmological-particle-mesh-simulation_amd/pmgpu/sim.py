@@ -84,6 +84,34 @@ class PMSim:
     def deposit(self):
         _lib.call("pm_sim_deposit", self._h)
 
+    def zeldovich(self, amp, psi_scale: float, vel_factor: float, seed: int = 0, noise=None):
+        """Device initial conditions: s = Psi, v = vel_factor * Psi (see pm_sim_zeldovich).
+
+        ``amp[m2]`` = sqrt(P(k) N_PARTS^3 / V) D(a) per integer |m|^2; ``noise`` is an optional
+        DeviceArray of N_PARTS^3 float32/float64 normal deviates in [i][j][k] order (otherwise
+        the Philox4x32-10 field of ``seed``)."""
+        amp = np.ascontiguousarray(amp, dtype=np.float64)
+        ptr, nb = None, 0
+        if noise is not None:
+            if noise.dtype not in (np.float32, np.float64) or noise.nbytes != self.n_p * noise.dtype.itemsize:
+                raise ValueError(f"noise must hold {self.n_p} float32/float64 values")
+            ptr, nb = noise.ptr, noise.dtype.itemsize
+        _lib.call("pm_sim_zeldovich", self._h, amp.ctypes.data_as(C.POINTER(C.c_double)), amp.size,
+                  float(psi_scale), float(vel_factor), int(seed) & (2 ** 64 - 1), ptr, nb)
+        self.version += 1
+
+    def power_spectrum(self, box: float, n_bins: int | None = None):
+        """(k in h/Mpc, P(k) in (Mpc/h)^3, mode counts) of the current density, unit shells in
+        |m| up to N/2 by default; computed on the device (FP64 mean and shell sums)."""
+        nb = int(n_bins or self.n_cells // 2)
+        k = np.zeros(nb)
+        pk = np.zeros(nb)
+        cnt = np.zeros(nb)
+        dp = C.POINTER(C.c_double)
+        _lib.call("pm_sim_power_spectrum", self._h, float(box), nb, k.ctypes.data_as(dp),
+                  pk.ctypes.data_as(dp), cnt.ctypes.data_as(dp))
+        return k * 2 * np.pi / box, pk, cnt
+
     def set_density(self, rho):
         rho = np.ascontiguousarray(rho)
         n = self.n_cells

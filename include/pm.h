@@ -131,6 +131,26 @@ PM_API int pm_sim_sync(pm_sim* sim);
 /* bytes of device memory held by the sim                                                     */
 PM_API int pm_sim_footprint(int n_cells, int n_parts_side, int precision, size_t* bytes);
 
+/* ------------------------------------------------ the step's two ends: ICs and the spectrum */
+/* gaussian_random_field.py:32-50 draws the field's normal deviates from an unseeded parallel
+ * RNG; here: element e = Philox4x32-10(counter e/4, key seed) + Box-Muller, n float64 values.  */
+PM_API int pm_white_noise(double* d_out, int64_t n, uint64_t seed);
+/* gaussian_random_field.py:9-29 + zeldovich.py:10-100 on the device: the sim's lattice (N_PARTS
+ * a supported FFT size below N_CELLS) gets s = Psi and v = vel_factor * Psi with
+ * Psi_d = psi_scale * IDFT(i m_d / |m|^2 * h_amp[|m|^2] * W), W the DFT of white noise:
+ * d_noise (N_PARTS^3 values in [i][j][k] order, float32 or float64 by noise_dtype_bytes, on
+ * the device) or, when NULL, pm_white_noise(seed).  h_amp holds n_amp >= 3 (N_PARTS/2)^2 + 1
+ * host doubles sqrt(P(k) N_PARTS^3 / V) D(a) per integer |m|^2.                             */
+PM_API int pm_sim_zeldovich(pm_sim* sim, const double* h_amp, int64_t n_amp, double psi_scale,
+                            double vel_factor, uint64_t seed, const void* d_noise,
+                            int noise_dtype_bytes);
+/* matter power spectrum of the sim's density (the analysis the reference leaves to its plots,
+ * plot_helper.py): P(k) = V / N^6 <|DFT(rho / <rho> - 1)|^2> over unit shells in |m|
+ * (bin floor(|m| + 1/2) = 1 .. n_bins), mean density reduced in FP64; outputs per bin the
+ * mean |m|, P and the mode count (host arrays of n_bins doubles).  Overwrites the potential. */
+PM_API int pm_sim_power_spectrum(pm_sim* sim, double box, int n_bins, double* h_kmode, double* h_pk,
+                                 double* h_counts);
+
 /* ------------------------------------------- batched ensemble (many sims, one launch/stage) */
 /* No single reference function corresponds: the reference runs one realisation per process
  * (pmesh.py:54-77).  An ensemble advances many FP32 sims, of grid sizes 32..512 (2^a 3^b,

@@ -38,6 +38,7 @@ __all__ = [
     "wrap_positions", "density", "fourier_grid", "density_k", "potential_k",
     "potential_real", "potential", "f", "H", "Dt", "cell_centers", "weights",
     "sweep_one_direction", "integrate", "advance_time", "step_count", "OracleParams",
+    "philox4x32_10", "white_noise",
     "run", "lattice_q", "power_spectrum",
 ]
 
@@ -351,3 +352,48 @@ def power_spectrum(rho: np.ndarray, box: float, n_bins: int | None = None):
     kc = np.where(good, ksum / np.maximum(cnt, 1), 0) * 2 * np.pi / box
     pk = np.where(good, psum / np.maximum(cnt, 1), 0) * box ** 3 / n ** 6
     return kc[1:], pk[1:], cnt[1:]
+
+
+# ------------------------------------------------------------------ seeded white noise
+# The reference draws its field's deviates from an unseeded parallel RNG
+# (gaussian_random_field.py:32-50), which no run can reproduce; the device generator replaces
+# it with a counter-based one, restated here.
+
+_M0, _M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+_W0, _W1 = 0x9E3779B9, 0xBB67AE85
+_MASK = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11), vectorised: ctr is (4, m) uint32
+    counters, key two uint32 words; returns (4, m) uint32.  Each round maps
+    (x0, x1, x2, x3) -> (hi(M1 x2) ^ x1 ^ k0, lo(M1 x2), hi(M0 x0) ^ x3 ^ k1, lo(M0 x0)) and
+    the key advances by (W0, W1) between rounds."""
+    c = [np.asarray(x, dtype=np.uint64) & _MASK for x in ctr]
+    k0, k1 = int(key[0]) & 0xFFFFFFFF, int(key[1]) & 0xFFFFFFFF
+    for r in range(10):
+        if r:
+            k0 = (k0 + _W0) & 0xFFFFFFFF
+            k1 = (k1 + _W1) & 0xFFFFFFFF
+        p0 = _M0 * c[0]
+        p1 = _M1 * c[2]
+        c = [((p1 >> np.uint64(32)) ^ c[1] ^ np.uint64(k0)) & _MASK, p1 & _MASK,
+             ((p0 >> np.uint64(32)) ^ c[3] ^ np.uint64(k1)) & _MASK, p0 & _MASK]
+    return np.stack(c).astype(np.uint32)
+
+
+def white_noise(n: int, seed: int) -> np.ndarray:
+    """n standard normal deviates: element e comes from counter (e // 4, 0, 0, 0), key
+    (seed low, seed high); uniforms (u + 1/2) 2^-32 and Box-Muller on the word pairs (0, 1)
+    -> (r cos, r sin) and (2, 3) -> (r cos, r sin)."""
+    groups = (n + 3) // 4
+    g = np.arange(groups, dtype=np.uint64)
+    ctr = [g & _MASK, g >> np.uint64(32), np.zeros_like(g), np.zeros_like(g)]
+    u = philox4x32_10(ctr, (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)).astype(np.float64)
+    s = 2.0 ** -32
+    r0 = np.sqrt(-2.0 * np.log((u[0] + 0.5) * s))
+    t0 = 2 * np.pi * ((u[1] + 0.5) * s)
+    r1 = np.sqrt(-2.0 * np.log((u[2] + 0.5) * s))
+    t1 = 2 * np.pi * ((u[3] + 0.5) * s)
+    z = np.stack([r0 * np.cos(t0), r0 * np.sin(t0), r1 * np.cos(t1), r1 * np.sin(t1)], axis=1)
+    return z.reshape(-1)[:n]

@@ -276,3 +276,39 @@ def test_power_spectrum_of_plane_wave():
     total = (pk * cnt).sum() * n ** 6 / box ** 3
     assert abs(total - 2 * (0.1 * n ** 3) ** 2) < 1e-6 * total
     assert np.argmax(pk) == 2
+
+
+# Philox4x32-10 known-answer vectors published with the Random123 library (Salmon et al.,
+# SC'11; kat_vectors, philox4x32 with 10 rounds): (counter, key) -> output
+PHILOX_KAT = [
+    ((0x00000000, 0x00000000, 0x00000000, 0x00000000), (0x00000000, 0x00000000),
+     (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff), (0xffffffff, 0xffffffff),
+     (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, expect):
+    out = O.philox4x32_10([[c] for c in ctr], key)[:, 0]
+    assert [int(x) for x in out] == list(expect)
+
+
+def test_white_noise_is_standard_normal_and_seeded():
+    n = 1_000_003                      # not a multiple of 4: the last group is partial
+    z = O.white_noise(n, 42)
+    assert z.shape == (n,)
+    assert abs(z.mean()) < 5 / math.sqrt(n)
+    assert abs(z.var() - 1) < 5 * math.sqrt(2 / n)
+    # tail fraction beyond 3 sigma: 0.27 %
+    assert abs(np.mean(np.abs(z) > 3) - 0.0026998) < 3e-4
+    np.testing.assert_array_equal(z[:1000], O.white_noise(1000, 42))
+    assert np.abs(z[:1000] - O.white_noise(1000, 43)).max() > 1
+    # element e is word e % 4 of counter e // 4: the first two come from one Box-Muller pair
+    u = O.philox4x32_10([[0], [0], [0], [0]], (42, 0))[:, 0].astype(np.float64)
+    r = math.sqrt(-2 * math.log((u[0] + 0.5) * 2.0 ** -32))
+    t = 2 * math.pi * (u[1] + 0.5) * 2.0 ** -32
+    assert z[0] == pytest.approx(r * math.cos(t), rel=1e-15)
+    assert z[1] == pytest.approx(r * math.sin(t), rel=1e-15)
