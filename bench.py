@@ -223,6 +223,100 @@ def ensemble_bench(e, steps=None):
             "speedup_vs_one_at_a_time": t_solo / t_ens}
 
 
+# Serving stream: Poisson arrivals at 2 requests/s for 600 s, N_grid in {64..384} with weights
+# 0.3/0.3/0.2/0.15/0.05, N_parts = N_grid/2, box U(50, 400), STEPS U{50..500}, sequential seeds.
+SERVE = dict(rate=2.0, duration=600.0, seed=4, sizes=(64, 128, 192, 256, 384),
+             weights=(0.3, 0.3, 0.2, 0.15, 0.05), box=(50.0, 400.0), steps=(50, 500))
+
+
+def serve_bench(sv, window, chunk_steps=16):
+    """The first `window` seconds of the serving stream, three ways: served in real time with
+    continuous batching (time to completion = completion - arrival); served with every request
+    available at once (capacity, sims/hour); and one request at a time back to back (its
+    capacity, and its time to completion under the same arrivals by FIFO replay of the
+    measured run times)."""
+    from pmgpu.serve import PMServer, SimRequest, latency_summary, poisson_stream, run_alone
+    reqs = poisson_stream(sv["rate"], window, seed=sv["seed"], sizes=sv["sizes"],
+                          weights=sv["weights"], box=sv["box"], steps=sv["steps"])
+    warm = [SimRequest(id=-1 - i, seed=1, box=100.0, n_cells=n, n_parts=n // 2, steps=20)
+            for i, n in enumerate(sv["sizes"])]
+    PMServer().serve(warm, realtime=False)
+    for r in warm:
+        run_alone(r)
+    updates = sum(r.n_parts ** 3 * r.loop_steps for r in reqs)
+    log(f"serving: {len(reqs)} requests over {window:.0f} s, real time")
+    server = PMServer(chunk_steps=chunk_steps)
+    res = server.serve(reqs, realtime=True)
+    live = dict(latency_summary(res), **server.stats)
+    log(f"serving: p50 {live['p50_s']:.3f} s p99 {live['p99_s']:.3f} s; all at once")
+    server = PMServer(chunk_steps=chunk_steps)
+    t0 = time.perf_counter()
+    server.serve(reqs, realtime=False)
+    t_batch = time.perf_counter() - t0
+    log(f"serving all at once: {t_batch:.2f} s; one at a time")
+    service = []
+    for r in reqs:
+        service.append(run_alone(r).finished)
+    t_solo = float(np.sum(service))
+    done, ttc = 0.0, []
+    for r, t in zip(reqs, service):                    # requests are in arrival order
+        start = max(r.arrival, done)
+        done = start + t
+        ttc.append(done - r.arrival)
+    log(f"one at a time: {t_solo:.2f} s; the pmesh.py way on a sample")
+    drv = driver_sample(reqs, per_size=2)
+    t_drv = sum(drv["mean_s"][r.n_cells] for r in reqs)
+    counts = {n: sum(1 for r in reqs if r.n_cells == n) for n in sv["sizes"]}
+    return {"workload": f"serve: Poisson {sv['rate']}/s over the first {window:.0f} s of the "
+                        f"stream (seed {sv['seed']}), N_grid counts {counts}, N_parts = N_grid/2, "
+                        f"box U{sv['box']}, STEPS U{sv['steps']}, device ICs, FP32",
+            "requests": len(reqs), "particle_updates": updates,
+            "realtime": {"p50_s": live["p50_s"], "p99_s": live["p99_s"], "mean_s": live["mean_s"],
+                         "wall_s": live["wall_s"], "device_busy_s": live["device_busy_s"],
+                         "peak_members": live["peak_members"]},
+            "capacity_sims_per_hour": len(reqs) / t_batch * 3600,
+            "capacity_particle_updates_per_s": updates / t_batch,
+            "one_at_a_time": {"sims_per_hour": len(reqs) / t_solo * 3600,
+                              "p50_s": float(np.percentile(ttc, 50)),
+                              "p99_s": float(np.percentile(ttc, 99))},
+            "capacity_vs_one_at_a_time": t_solo / t_batch,
+            "one_at_a_time_pmesh_driver": {
+                "sims_per_hour": len(reqs) / t_drv * 3600,
+                "how": "each request as its own pmesh.py-style run: host initial conditions "
+                       "(NumPy), resident sim stepped once per loop iteration from Python, "
+                       "device P(k); time per N_grid from the first "
+                       f"{drv['per_size']} requests of each size, summed over the stream",
+                "mean_s_by_n_grid": drv["mean_s"]},
+            "capacity_vs_pmesh_driver": t_drv / t_batch}
+
+
+def driver_sample(reqs, per_size):
+    """Run time of requests run the way pmesh.py runs one simulation (host ICs, one Python
+    call per step with the state resident), on the first `per_size` requests of each size."""
+    import contextlib
+    import initial_conditions as ic
+    from pmgpu.sim import PMSim
+    times = {}
+    for r in reqs:
+        if len(times.setdefault(r.n_cells, [])) >= per_size:
+            continue
+        t0 = time.perf_counter()
+        p = r.ic_params()
+        with contextlib.redirect_stdout(sys.stderr):
+            pos, vel = ic.zeldovich_numpy(ic.gaussian_random_field(p), p)
+        sim = PMSim(r.n_cells, r.n_parts, 32, omega_m0=r.om, omega_lambda0=r.ol, omega_k0=r.ok,
+                    h0=r.h)
+        sim.set_state(pos, vel)
+        a, da = r.a_init, r.da
+        while a < r.a_end - da:                   # pmesh.py:56-63
+            sim.step(a, da)
+            a += da
+        sim.power_spectrum(r.box)
+        sim.close()
+        times[r.n_cells].append(time.perf_counter() - t0)
+    return {"per_size": per_size, "mean_s": {n: float(np.mean(t)) for n, t in times.items()}}
+
+
 def pk_error(sim32, sim64, box):
     """max |P32/P64 - 1| over k < k_Nyquist/2, both spectra from the device estimator."""
     _, p32, _ = sim32.power_spectrum(box)
@@ -289,6 +383,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-window", type=float, default=60.0,
+                    help="seconds of the 600 s serving stream to replay (600 = all of it)")
     ap.add_argument("--ensemble-steps", type=int, default=None)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a known-size kernel for "
@@ -371,6 +468,7 @@ def main():
     out["plain_fp64"] = plain_fp64_bench(pos, vel, w, args.plain_steps)
     out["fp32_over_plain_fp64"] = v32 / out["plain_fp64"]["value"]
     out["ensemble"] = None if args.no_ensemble else ensemble_bench(ENSEMBLE, args.ensemble_steps)
+    out["serve"] = None if args.no_serve else serve_bench(SERVE, args.serve_window)
     if not args.no_cpu:
         log("CPU baseline (port) on host cores")
         out["cpu_baseline"] = cpu_baseline(pos, vel, w, args.cpu_steps, args.cpu_budget)

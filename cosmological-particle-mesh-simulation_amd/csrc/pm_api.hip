@@ -206,6 +206,13 @@ PM_API int pm_memset(void* dst, int value, size_t bytes) {
   if (bytes) PM_HIP(hipMemset(dst, value, bytes));
   return PM_OK;
 }
+PM_API int pm_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  PM_INIT();
+  if (!free_bytes || !total_bytes) return fail(PM_E_INVALID, "null argument");
+  PM_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return PM_OK;
+}
+
 PM_API int pm_synchronize(void) {
   PM_INIT();
   PM_HIP(hipDeviceSynchronize());
@@ -1330,7 +1337,9 @@ PM_API int pm_ensemble_run(pm_ensemble* e, double* a, const double* da, int n_st
   // the launch shapes only change with the membership
   ens_describe(e);
   PM_HIP(hipMemcpyAsync(e->d_desc, e->host, sizeof(BatchDesc), hipMemcpyHostToDevice, e->stream));
-  const bool use_graph = !e->timing && n_steps >= 2;
+  // A graph pays for its capture over many replays; right after a membership change a short
+  // call is cheaper enqueued launch by launch.
+  const bool use_graph = !e->timing && n_steps >= kGraphSteps && (e->graph || n_steps >= 8 * kGraphSteps);
   if (use_graph)
     if (int rc = ens_build_graph(e)) return rc;
   std::vector<double> acur(a, a + nm);

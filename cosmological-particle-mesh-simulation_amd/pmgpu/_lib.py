@@ -40,6 +40,7 @@ SIGNATURES = {
     "pm_memcpy_dtod": (_i, [_vp, _vp, _sz]),
     "pm_memset": (_i, [_vp, _i, _sz]),
     "pm_synchronize": (_i, []),
+    "pm_mem_info": (_i, [C.POINTER(_sz), C.POINTER(_sz)]),
     "pm_density": (_i, [_vp, _i64, _i, _d, _vp]),
     "pm_fourier_grid": (_i, [_i, _vp]),
     "pm_density_k": (_i, [_vp, _i, _vp]),

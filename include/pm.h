@@ -52,6 +52,7 @@ PM_API int pm_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
 PM_API int pm_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
 PM_API int pm_memset(void* d_dst, int value, size_t bytes);
 PM_API int pm_synchronize(void);
+PM_API int pm_mem_info(size_t* free_bytes, size_t* total_bytes);  /* device HBM, for admission */
 
 /* ------------------------------------------- reference-function mirrors (device pointers) */
 /* density.py:7-48  density(positions, mass): CIC deposit into a new float32 N^3 grid.       */

@@ -54,12 +54,15 @@ def test_ensemble_matches_members_run_alone():
     assert len(ens) == len(MEMBERS)
     a0 = [m[2] for m in MEMBERS]
     da = [m[3] for m in MEMBERS]
-    steps = 11   # one graph replay of 8 steps plus 3 table-driven steps
-    a_out = ens.run(a0, da, steps)
-    for i, (alone, member) in enumerate(pairs):
-        a_alone = alone.run(a0[i], da[i], steps)
-        assert a_out[i] == a_alone
-        _assert_same(alone, member, f"member {i}")
+    # 11 steps: enqueued launch by launch (too short to capture a graph); then 70 steps: a
+    # captured graph of 8 steps replayed 8 times plus 6 table-driven steps
+    for steps in (11, 70):
+        a_out = ens.run(a0, da, steps)
+        for i, (alone, member) in enumerate(pairs):
+            a_alone = alone.run(a0[i], da[i], steps)
+            assert a_out[i] == a_alone
+            _assert_same(alone, member, f"{steps} steps, member {i}")
+        a0 = a_out
     # the potential of the last step is kept per member as well
     for alone, member in pairs:
         assert rel(member.potential(np.float64), alone.potential(np.float64)) < 1e-5
@@ -105,21 +108,25 @@ def test_ensemble_membership_changes_and_solo_steps():
             s.close()
 
 
-def test_ensemble_eager_timing_path_equals_graph():
+def test_ensemble_graph_replays_equal_eager_steps():
+    # the captured graph is replayed in three calls, with eager work and synchronisation
+    # between them; every call must agree with the same steps enqueued launch by launch
     from pmgpu.ensemble import PMEnsemble
     p1 = [_pair(i, seed0=80) for i in (0, 2)]
     p2 = [_pair(i, seed0=80) for i in (0, 2)]
     e1 = PMEnsemble([m for _, m in p1])
     e2 = PMEnsemble([m for _, m in p2])
-    e2.set_timing(True)
-    a, da = [0.02, 0.05], [0.01, 0.005]
-    a1 = e1.run(a, da, 10)
-    a2 = e2.run(a, da, 10)
-    assert a1 == a2
+    e2.set_timing(True)                                 # timing: launch by launch, no graph
+    a, da = [0.02, 0.05], [0.001, 0.0005]
+    for rep, steps in enumerate((70, 16, 9)):
+        a1 = e1.run(a, da, steps)
+        a2 = e2.run(a, da, steps)
+        assert a1 == a2
+        for (_, m1), (_, m2) in zip(p1, p2):
+            _assert_same(m1, m2, f"graph vs eager, call {rep}")
+        a = a1
     t = e2.timing()
-    assert t["z_green"][1] == 10 and t["gather_kick_drift"][0] > 0
-    for (_, m1), (_, m2) in zip(p1, p2):
-        _assert_same(m1, m2, "eager vs graph")
+    assert t["z_green"][1] == 95 and t["gather_kick_drift"][0] > 0
     e1.close()
     e2.close()
 
