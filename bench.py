@@ -366,7 +366,8 @@ def traffic_from_profiles(kernel):
     if not f.exists():
         return None
     try:
-        return json.loads(f.read_text()).get(kernel)
+        d = json.loads(f.read_text()).get(kernel)
+        return None if d is None else float(d["bytes"])
     except Exception:
         return None
 
