@@ -74,6 +74,81 @@ template <int RAD, int S, typename T> __device__ __forceinline__ void dftR(T* v)
   else static_assert(RAD == 2, "unsupported radix");
 }
 
+// ------------------------------------------------------------------------------------------
+// DFTs held entirely in registers (sizes 2^a 3^b up to a few dozen), for the four-step pencil
+// transform.  Twiddle constants are evaluated at compile time.
+// ------------------------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846;
+
+// sin / cos of x in [-pi, pi] by their Taylor series (converged to double precision)
+constexpr double c_sin(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 30; ++k) {
+    term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double c_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 30; ++k) {
+    term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+
+// exp(-2 pi i m / A) for m in [0, A)
+template <int A> struct RegTwiddles {
+  double re[A], im[A];
+  constexpr RegTwiddles() : re(), im() {
+    for (int m = 0; m < A; ++m) {
+      double ang = 2.0 * kPi * m / A;
+      if (ang > kPi) ang -= 2.0 * kPi;
+      re[m] = c_cos(ang);
+      im[m] = -c_sin(ang);
+    }
+  }
+};
+
+// DFT of A values in registers, sign S: decimation in time, A = M * RR with the RR-point
+// butterflies last: X[k1 + M k2] = sum_r W_RR^{r k2} W_A^{r k1} DFT_M(x[m RR + r])[k1].
+template <int S, int A, typename T> __device__ __forceinline__ void reg_dft(T (&v)[A]) {
+  using R = decltype(T::x);
+  if constexpr (A == 1) {
+  } else if constexpr (A == 2 || A == 3 || A == 4 || A == 8) {
+    dftR<A, S>(v);
+  } else {
+    constexpr int RR = (A % 8 == 0) ? 8 : (A % 4 == 0) ? 4 : (A % 3 == 0) ? 3 : 2;
+    constexpr int M = A / RR;
+    constexpr RegTwiddles<A> tw{};
+    T sub[RR][M];
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+#pragma unroll
+      for (int m = 0; m < M; ++m) sub[r][m] = v[m * RR + r];
+#pragma unroll
+    for (int r = 0; r < RR; ++r) reg_dft<S, M>(sub[r]);
+#pragma unroll
+    for (int r = 1; r < RR; ++r)
+#pragma unroll
+      for (int k1 = 1; k1 < M; ++k1) {
+        const int idx = (r * k1) % A;
+        const T w = cmake<T>(R(tw.re[idx]), R(S < 0 ? tw.im[idx] : -tw.im[idx]));
+        sub[r][k1] = cmul(sub[r][k1], w);
+      }
+#pragma unroll
+    for (int k1 = 0; k1 < M; ++k1) {
+      T col[RR];
+#pragma unroll
+      for (int r = 0; r < RR; ++r) col[r] = sub[r][k1];
+      dftR<RR, S>(col);
+#pragma unroll
+      for (int k2 = 0; k2 < RR; ++k2) v[k1 + M * k2] = col[k2];
+    }
+  }
+}
+
 // largest preferred radix dividing the remaining length
 template <int L> struct first_radix {
   static constexpr int value = (L % 8 == 0) ? 8 : (L % 4 == 0) ? 4 : (L % 2 == 0) ? 2 : (L % 3 == 0) ? 3 : 0;

@@ -240,108 +240,303 @@ template <typename R, int N> constexpr int pencil_lds() {
   return (N * (pencil_w<R, N>() + 1) + N) * (int)sizeof(cpx<R>) + 2 * N * (int)sizeof(R);
 }
 
-// MODE: -1 forward only, +1 inverse only, 0 forward + k-space operator + inverse
+// the k-space operator's scalar prefactor of this step
+__device__ __forceinline__ double coef_of(const GreenArgs& g) { return g.table ? g.table[3 * (*g.step)] : g.coef; }
+
+// v[m] *= W_N^{S f m} for m < M, from 8 + M/8 table reads: W^{f (8 hi + lo)} = W^{8 f hi} W^{f lo}
+// (each factor is one table entry, so every twiddle carries one extra rounding at most)
+template <int S, int N, int M, int G, typename T>
+__device__ __forceinline__ void twiddle_row(T (&v)[M], const T* tw, int f) {
+  constexpr int LO = (M < G ? M : G);
+  T lo[LO];
+#pragma unroll
+  for (int r = 1; r < LO; ++r) lo[r] = tw[(f * r) % N];
+#pragma unroll
+  for (int hi = 0; hi < M; hi += LO) {
+    const T h = tw[(f * hi) % N];
+#pragma unroll
+    for (int r = 0; r < LO; ++r) {
+      if (hi == 0 && r == 0) continue;
+      const T c = hi == 0 ? lo[r] : (r == 0 ? h : cmul(h, lo[r]));
+      v[hi + r] = S < 0 ? cmul(v[hi + r], c) : cmulc(v[hi + r], c);
+    }
+  }
+}
+
+// Where pencil `blk` of a pass lives: its first element, the strides, and (for the k-space
+// operator of the z pass) the kx / ky of its columns.
+template <typename R> struct PencilAddr {
+  cpx<R>* base;
+  long stride_n, stride_w;
+  int contig;
+  int kx0, dkx, ky0, dky;
+  long tail;  // elements of the array from base to its end
+};
+
+template <typename R, int N>
+__device__ __forceinline__ PencilAddr<R> pencil_addr(cpx<R>* H, cpx<R>* Q, const PencilSeg& s0, const PencilSeg& s1,
+                                                     long blk) {
+  constexpr int W = pencil_w<R, N>();
+  const bool first = blk < s0.nblk;
+  const PencilSeg& s = first ? s0 : s1;
+  if (!first) blk -= s0.nblk;
+  cpx<R>* data = first ? H : Q;
+  PencilAddr<R> p{};
+  p.contig = s.contig;
+  p.stride_n = s.stride_n;
+  p.stride_w = s.stride_w;
+  if (s.contig) {
+    const long o = blk / s.ncol_blk, c = blk - o * s.ncol_blk;
+    p.base = data + o * s.outer_stride + c * W;
+    if (s.green_kind == 0) {
+      p.kx0 = (int)(c * W);
+      p.dkx = 1;
+      p.ky0 = (int)o;
+    } else {
+      p.kx0 = N / 2;
+      p.ky0 = (int)(c * W);
+      p.dky = 1;
+    }
+  } else {
+    p.base = data + blk * (long)W * s.stride_w;
+  }
+  p.tail = (first ? (long)N * N * (N / 2) : (long)N * N) - (p.base - data);
+  return p;
+}
+
+// Raw buffer access for the register pass: the resource holds the pencil's base and the bytes
+// to the end of its array (out-of-range accesses read 0 and write nothing), so each element's
+// address is a per-thread VGPR offset plus a block-uniform SGPR offset.
+using u32x2 = unsigned int __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* base, long bytes) {
+  // 0x00020000: the buffer-resource format word of the gfx9 family (raw 32-bit dwords)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
+                                           (int)(bytes < 0x7fffffffL ? bytes : 0x7fffffffL), 0x00020000);
+}
+__device__ __forceinline__ cpx<float> buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, int voff, int soff, cpx<float> v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+// elements of one pencil per thread
+template <typename R, int N> constexpr int pencil_lpt() { return (N * pencil_w<R, N>() + kThreads - 1) / kThreads; }
+
+// element i of this thread: (row n, column w) of the pencil; contiguous pencils walk along w
+template <typename R, int N>
+__device__ __forceinline__ bool pencil_elem(int contig, int i, int tid, int& n, int& w) {
+  constexpr int W = pencil_w<R, N>();
+  const int e = i * kThreads + tid;
+  if (contig) {
+    n = e / W;
+    w = e - n * W;
+  } else {
+    w = e / N;
+    n = e - w * N;
+  }
+  return e < N * W;
+}
+
+// global -> LDS, in chunks of CH loads per thread issued before any is stored.  In a
+// contiguous pencil thread tid always handles column tid % W, and element i sits (NT / W) * i
+// rows below its first.
+template <typename R, int N>
+__device__ __forceinline__ void pencil_load(const PencilAddr<R>& p, cpx<R>* lds, int tid) {
+  constexpr int W = pencil_w<R, N>();
+  constexpr int SN = W + 1;
+  constexpr int LPT = pencil_lpt<R, N>();
+  constexpr int CH = sizeof(R) == 4 ? 32 : 8;
+  if (p.contig) {
+    const cpx<R>* src = p.base + (tid / W) * p.stride_n + (tid % W);
+    cpx<R>* dst = lds + (tid / W) * SN + (tid % W);
+    const long step = (long)(kThreads / W) * p.stride_n;
+#pragma unroll
+    for (int i0 = 0; i0 < LPT; i0 += CH) {
+      cpx<R> u[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        if (i0 + i < LPT && ((N * W) % kThreads == 0 || (i0 + i) * kThreads + tid < N * W))
+          u[i] = src[(i0 + i) * step];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        if (i0 + i < LPT && ((N * W) % kThreads == 0 || (i0 + i) * kThreads + tid < N * W))
+          dst[(i0 + i) * (kThreads / W) * SN] = u[i];
+    }
+  } else {
+#pragma unroll
+    for (int i0 = 0; i0 < LPT; i0 += CH) {
+      cpx<R> u[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        int n, w;
+        if (i0 + i < LPT && pencil_elem<R, N>(0, i0 + i, tid, n, w)) u[i] = p.base[n + w * p.stride_w];
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        int n, w;
+        if (i0 + i < LPT && pencil_elem<R, N>(0, i0 + i, tid, n, w)) lds[n * SN + w] = u[i];
+      }
+    }
+  }
+}
+
+// Four-step transform of the staged pencil with the columns' DFTs in registers: N = A * TT,
+// TT = threads per column.  Thread t of a column takes x[a TT + t] (a < A), does the A-point
+// DFT and the twiddle W_N^{t ka}; after one exchange through LDS it takes, for each of its
+// J = A / TT values ka = t + j TT, the TT values Y[t'][ka] and does the TT-point DFT, giving
+// X[ka + A kb].  The inverse of the fused z pass runs the same steps backwards from there.
+template <typename R, int N> constexpr bool reg_transform() {
+  constexpr int W = pencil_w<R, N>();
+  constexpr int TT = kThreads / W;
+  // (the raw-buffer addressing of the register pass covers arrays below 2 GiB: N <= 768)
+  return sizeof(R) == 4 && kThreads % W == 0 && N % TT == 0 && (N / TT) % TT == 0 && (TT == 16 || TT == 32) &&
+         N / TT <= 32 && N <= 768;
+}
+
 template <typename R, int N, int MODE>
-__device__ __forceinline__ void pencil_body(cpx<R>* H, cpx<R>* Q, const PencilSeg& s0, const PencilSeg& s1,
-                                            const GreenArgs& g, const cpx<R>* __restrict__ twg,
-                                            const R* __restrict__ s2g, const R* __restrict__ wdg, long blk,
-                                            cpx<R>* lds, cpx<R>* tw, R* s2, R* wd) {
+__device__ __forceinline__ void pencil_transform_reg(const PencilAddr<R>& p, const GreenArgs& g, R coef, cpx<R>* lds,
+                                                     const cpx<R>* tw, const R* s2, const R* wd, int tid) {
+  using T = cpx<R>;
+  constexpr int W = pencil_w<R, N>();
+  constexpr int SN = W + 1;
+  constexpr int TT = kThreads / W;
+  constexpr int A = N / TT;
+  constexpr int J = A / TT;
+  constexpr int S1 = MODE == 1 ? +1 : -1;
+  const int w = tid % W, t = tid / W;
+  T v[A];
+#pragma unroll
+  for (int a = 0; a < A; ++a) v[a] = lds[(a * TT + t) * SN + w];
+  reg_dft<S1, A>(v);
+#pragma unroll
+  for (int ka = 1; ka < A; ++ka) {
+    const T c = tw[(t * ka) % N];
+    v[ka] = S1 < 0 ? cmul(v[ka], c) : cmulc(v[ka], c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int ka = 0; ka < A; ++ka) lds[(ka * TT + t) * SN + w] = v[ka];
+  __syncthreads();
+  T u[J][TT];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int ka = t + j * TT;
+#pragma unroll
+    for (int q = 0; q < TT; ++q) u[j][q] = lds[(ka * TT + q) * SN + w];
+    reg_dft<S1, TT>(u[j]);
+  }
+  if constexpr (MODE != 0) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+      for (int kb = 0; kb < TT; ++kb) lds[(t + j * TT + A * kb) * SN + w] = u[j][kb];
+    __syncthreads();
+  } else {
+    // k-space operator on X[k], k = ka + A kb the z mode, then the inverse back to x
+    const int kx = p.kx0 + w * p.dkx, ky = p.ky0 + w * p.dky;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int ka = t + j * TT;
+#pragma unroll
+      for (int kb = 0; kb < TT; ++kb) u[j][kb] = cscale(u[j][kb], green_factor<R>(kx, ky, ka + A * kb, N, g, coef, s2, wd));
+      reg_dft<+1, TT>(u[j]);
+      twiddle_row<+1, N, TT, 8>(u[j], tw, ka);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+      for (int q = 0; q < TT; ++q) lds[((t + j * TT) * TT + q) * SN + w] = u[j][q];
+    __syncthreads();
+#pragma unroll
+    for (int ka = 0; ka < A; ++ka) v[ka] = lds[(ka * TT + t) * SN + w];
+    reg_dft<+1, A>(v);
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < A; ++a) lds[(a * TT + t) * SN + w] = v[a];
+    __syncthreads();
+  }
+}
+
+// the transform(s) of a staged pencil, in LDS (ends with the LDS holding the result)
+template <typename R, int N, int MODE>
+__device__ __forceinline__ void pencil_transform(const PencilAddr<R>& p, const GreenArgs& g, R coef, cpx<R>* lds,
+                                                 const cpx<R>* tw, const R* s2, const R* wd, int tid) {
   using T = cpx<R>;
   constexpr int L = N;
   constexpr int W = pencil_w<R, N>();
-  constexpr int SN = W + 1;  // room for either layout
+  constexpr int SN = W + 1;
   constexpr int NT = kThreads;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += NT) {
+  if constexpr (reg_transform<R, N>()) {
+    pencil_transform_reg<R, N, MODE>(p, g, coef, lds, tw, s2, wd, tid);
+  } else if constexpr (MODE == -1) {
+    fft_passes<T, L, 1, -1, W, SN, NT, 1>(lds, tw, tid);
+  } else if constexpr (MODE == 1) {
+    fft_passes<T, L, 1, +1, W, SN, NT, 1>(lds, tw, tid);
+  } else {
+    fft_passes<T, L, 1, -1, W, SN, NT, 1>(lds, tw, tid);
+#pragma unroll 4
+    for (int e = tid; e < L * W; e += NT) {
+      const int n = e / W, w = e - n * W;
+      const R f = green_factor<R>(p.kx0 + w * p.dkx, p.ky0 + w * p.dky, n, N, g, coef, s2, wd);
+      lds[n * SN + w] = cscale(lds[n * SN + w], f);
+    }
+    __syncthreads();
+    fft_passes<T, L, 1, +1, W, SN, NT, 1>(lds, tw, tid);
+  }
+}
+
+// LDS -> global
+template <typename R, int N>
+__device__ __forceinline__ void pencil_store(const PencilAddr<R>& p, const cpx<R>* lds, int tid) {
+  constexpr int W = pencil_w<R, N>();
+  constexpr int SN = W + 1;
+  constexpr int LPT = pencil_lpt<R, N>();
+  if (p.contig) {
+    cpx<R>* dst = p.base + (tid / W) * p.stride_n + (tid % W);
+    const cpx<R>* src = lds + (tid / W) * SN + (tid % W);
+    const long step = (long)(kThreads / W) * p.stride_n;
+#pragma unroll 8
+    for (int i = 0; i < LPT; ++i)
+      if ((N * W) % kThreads == 0 || i * kThreads + tid < N * W) dst[i * step] = src[i * (kThreads / W) * SN];
+  } else {
+#pragma unroll 4
+    for (int i = 0; i < LPT; ++i) {
+      int n, w;
+      if (pencil_elem<R, N>(0, i, tid, n, w)) p.base[n + w * p.stride_w] = lds[n * SN + w];
+    }
+  }
+}
+
+template <typename R, int N, int MODE>
+__device__ __forceinline__ void pencil_tables(const cpx<R>* __restrict__ twg, const R* __restrict__ s2g,
+                                              const R* __restrict__ wdg, cpx<R>* tw, R* s2, R* wd, int tid) {
+  for (int i = tid; i < N; i += kThreads) {
     tw[i] = twg[i];
     if constexpr (MODE == 0) {
       s2[i] = s2g[i];
       wd[i] = wdg[i];
     }
   }
-  const bool first = blk < s0.nblk;
-  const PencilSeg& s = first ? s0 : s1;
-  if (!first) blk -= s0.nblk;
-  T* data = first ? H : Q;
-  T* base;
-  int kx0 = 0, dkx = 0, ky0 = 0, dky = 0;
-  if (s.contig) {
-    const long o = blk / s.ncol_blk, c = blk - o * s.ncol_blk;
-    base = data + o * s.outer_stride + c * W;
-    if (s.green_kind == 0) { kx0 = (int)(c * W); dkx = 1; ky0 = (int)o; }
-    else { kx0 = N / 2; ky0 = (int)(c * W); dky = 1; }
-  } else {
-    base = data + blk * (long)W * s.stride_w;
-  }
-  // load: the block's loads are issued in chunks of CH per thread before they are stored
-  constexpr int LPT = (L * W + NT - 1) / NT;
-  constexpr int CH = sizeof(R) == 4 ? 32 : 8;
-  if (s.contig) {
-#pragma unroll
-    for (int i0 = 0; i0 < LPT; i0 += CH) {
-      T u[CH];
-#pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        const int e = (i0 + i) * NT + tid;
-        const int n = e / W, w = e - n * W;
-        if (i0 + i < LPT && e < L * W) u[i] = base[n * s.stride_n + w];
-      }
-#pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        const int e = (i0 + i) * NT + tid;
-        const int n = e / W, w = e - n * W;
-        if (i0 + i < LPT && e < L * W) lds[n * SN + w] = u[i];
-      }
-    }
-  } else {
-#pragma unroll
-    for (int i0 = 0; i0 < LPT; i0 += CH) {
-      T u[CH];
-#pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        const int e = (i0 + i) * NT + tid;
-        const int w = e / L, n = e - w * L;
-        if (i0 + i < LPT && e < L * W) u[i] = base[n + w * s.stride_w];
-      }
-#pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        const int e = (i0 + i) * NT + tid;
-        const int w = e / L, n = e - w * L;
-        if (i0 + i < LPT && e < L * W) lds[n * SN + w] = u[i];
-      }
-    }
-  }
+}
+
+// One pencil: fetch, transform, store.  MODE: -1 forward only, +1 inverse only, 0 forward +
+// k-space operator + inverse.
+template <typename R, int N, int MODE>
+__device__ __forceinline__ void pencil_body(cpx<R>* H, cpx<R>* Q, const PencilSeg& s0, const PencilSeg& s1,
+                                            const GreenArgs& g, const cpx<R>* __restrict__ twg,
+                                            const R* __restrict__ s2g, const R* __restrict__ wdg, long blk,
+                                            cpx<R>* lds, cpx<R>* tw, R* s2, R* wd) {
+  const int tid = threadIdx.x;
+  pencil_tables<R, N, MODE>(twg, s2g, wdg, tw, s2, wd, tid);
+  const PencilAddr<R> p = pencil_addr<R, N>(H, Q, s0, s1, blk);
+  pencil_load<R, N>(p, lds, tid);
   __syncthreads();
-  if constexpr (MODE == -1) {
-    fft_passes<T, L, 1, -1, W, SN, NT, 1>(lds, tw, tid);
-  } else if constexpr (MODE == 1) {
-    fft_passes<T, L, 1, +1, W, SN, NT, 1>(lds, tw, tid);
-  } else {
-    fft_passes<T, L, 1, -1, W, SN, NT, 1>(lds, tw, tid);
-    const R coef = R(g.table ? g.table[3 * (*g.step)] : g.coef);
-#pragma unroll 4
-    for (int e = tid; e < L * W; e += NT) {
-      const int n = e / W, w = e - n * W;
-      const R f = green_factor<R>(kx0 + w * dkx, ky0 + w * dky, n, N, g, coef, s2, wd);
-      lds[n * SN + w] = cscale(lds[n * SN + w], f);
-    }
-    __syncthreads();
-    fft_passes<T, L, 1, +1, W, SN, NT, 1>(lds, tw, tid);
-  }
-  // store
-  if (s.contig) {
-#pragma unroll 4
-    for (int e = tid; e < L * W; e += NT) {
-      const int n = e / W, w = e - n * W;
-      base[n * s.stride_n + w] = lds[n * SN + w];
-    }
-  } else {
-#pragma unroll 4
-    for (int e = tid; e < L * W; e += NT) {
-      const int w = e / L, n = e - w * L;
-      base[n + w * s.stride_w] = lds[n * SN + w];
-    }
-  }
+  const R coef = MODE == 0 ? R(g.table ? g.table[3 * (*g.step)] : g.coef) : R(0);
+  pencil_transform<R, N, MODE>(p, g, coef, lds, tw, s2, wd, tid);
+  pencil_store<R, N>(p, lds, tid);
 }
 
 template <typename R, int N, int MODE>
@@ -353,6 +548,100 @@ __global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, Penci
   __shared__ R s2[MODE == 0 ? N : 1];
   __shared__ R wd[MODE == 0 ? N : 1];
   pencil_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, blockIdx.x, lds, tw, s2, wd);
+}
+
+// Register pass: the four-step transform with the columns read from and written to global
+// memory directly, and only the exchange between its two DFT stages going through LDS, one
+// slice of TT ka values at a time (TT*TT rows of W columns), so a block holds a fraction of its
+// pencil in LDS and more blocks stay resident.  Row r of a slice sits at r*SN + r/TT + w: the
+// extra word per TT rows spreads both the writers' and the readers' lanes over the banks.
+template <typename R, int N> constexpr int xslice_elems() {
+  constexpr int W = pencil_w<R, N>();
+  constexpr int TT = kThreads / W;
+  return TT * TT * (W + 1) + TT;
+}
+
+template <typename R, int N, int MODE>
+__device__ __forceinline__ void pencil_reg_body(cpx<R>* H, cpx<R>* Q, const PencilSeg& s0, const PencilSeg& s1,
+                                                const GreenArgs& g, const cpx<R>* __restrict__ twg,
+                                                const R* __restrict__ s2g, const R* __restrict__ wdg, long blk,
+                                                cpx<R>* xb, cpx<R>* tw, R* s2, R* wd) {
+  using T = cpx<R>;
+  constexpr int W = pencil_w<R, N>();
+  constexpr int SN = W + 1;
+  constexpr int TT = kThreads / W;
+  constexpr int A = N / TT;
+  constexpr int J = A / TT;
+  constexpr int S1 = MODE == 1 ? +1 : -1;
+  const int tid = threadIdx.x;
+  const int w = tid % W, t = tid / W;
+  pencil_tables<R, N, MODE>(twg, s2g, wdg, tw, s2, wd, tid);
+  const PencilAddr<R> p = pencil_addr<R, N>(H, Q, s0, s1, blk);
+  // element (n, w) of the pencil at byte n * sn8 + (w * sw) * 8 from its base
+  const int sn8 = (int)((p.contig ? p.stride_n : 1) * (long)sizeof(T));
+  const __amdgpu_buffer_rsrc_t buf = buffer_of(p.base, p.tail * (long)sizeof(T));
+  const int voff = t * sn8 + w * (int)((p.contig ? 1 : p.stride_w) * (long)sizeof(T));
+  auto xrow = [&](int r) { return r * SN + r / TT + w; };
+  T v[A];
+#pragma unroll
+  for (int a = 0; a < A; ++a) v[a] = buffer_load(buf, voff, a * TT * sn8);
+  __syncthreads();  // tables in LDS
+  reg_dft<S1, A>(v);
+  twiddle_row<S1, N, A, 8>(v, tw, t);
+  T u[J][TT];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    if (j) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TT; ++q) xb[xrow(q * TT + t)] = v[j * TT + q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TT; ++q) u[j][q] = xb[xrow(t * TT + q)];
+    reg_dft<S1, TT>(u[j]);  // u[j][kb] = X[ka + A kb], ka = j TT + t
+  }
+  if constexpr (MODE != 0) {
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+      for (int kb = 0; kb < TT; ++kb) buffer_store(buf, voff, (j * TT + A * kb) * sn8, u[j][kb]);
+  } else {
+    // k-space operator on X[k], k = ka + A kb the z mode, then the same steps backwards
+    const int kx = p.kx0 + w * p.dkx, ky = p.ky0 + w * p.dky;
+    const R coef = R(coef_of(g));
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int ka = j * TT + t;
+#pragma unroll
+      for (int kb = 0; kb < TT; ++kb)
+        u[j][kb] = cscale(u[j][kb], green_factor<R>(kx, ky, ka + A * kb, N, g, coef, s2, wd));
+      reg_dft<+1, TT>(u[j]);
+#pragma unroll
+      for (int q = 1; q < TT; ++q) u[j][q] = cmulc(u[j][q], tw[(q * ka) % N]);
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < TT; ++q) xb[xrow(t * TT + q)] = u[j][q];
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < TT; ++r) v[j * TT + r] = xb[xrow(r * TT + t)];
+    }
+    reg_dft<+1, A>(v);
+#pragma unroll
+    for (int a = 0; a < A; ++a) buffer_store(buf, voff, a * TT * sn8, v[a]);
+  }
+}
+
+template <typename R, int N, int MODE>
+__global__ __launch_bounds__(kThreads) void k_pencil_reg(cpx<R>* H, cpx<R>* Q, PencilSeg s0, PencilSeg s1, GreenArgs g,
+                                                         const cpx<R>* __restrict__ twg, const R* __restrict__ s2g,
+                                                         const R* __restrict__ wdg) {
+  __shared__ cpx<R> xb[xslice_elems<R, N>()];
+  __shared__ cpx<R> tw[N];
+  __shared__ R s2[MODE == 0 ? N : 1];
+  __shared__ R wd[MODE == 0 ? N : 1];
+  pencil_reg_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, blockIdx.x, xb, tw, s2, wd);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -411,8 +700,13 @@ hipError_t launch_pencil(cpx<R>* H, cpx<R>* Q, bool zaxis, const GreenArgs& g, c
   PencilSeg a, b;
   if (zaxis) z_segments<R, N>(a, b);
   else y_segments<R, N>(a, b);
-  k_pencil<R, N, MODE><<<a.nblk + b.nblk, kThreads, 0, st>>>(H, Q, a, b, g, (const cpx<R>*)t.tw,
-                                                            (const R*)t.sin2, (const R*)t.wdec);
+  if constexpr (reg_transform<R, N>()) {
+    k_pencil_reg<R, N, MODE><<<a.nblk + b.nblk, kThreads, 0, st>>>(H, Q, a, b, g, (const cpx<R>*)t.tw,
+                                                                  (const R*)t.sin2, (const R*)t.wdec);
+  } else {
+    k_pencil<R, N, MODE><<<a.nblk + b.nblk, kThreads, 0, st>>>(H, Q, a, b, g, (const cpx<R>*)t.tw,
+                                                              (const R*)t.sin2, (const R*)t.wdec);
+  }
   return hipGetLastError();
 }
 

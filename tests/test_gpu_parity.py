@@ -291,3 +291,43 @@ def test_sim_rejects_bad_input():
     with pytest.raises(ValueError):
         s.set_state(np.zeros((3, 10), np.float32), np.zeros((3, 10), np.float32))
     s.close()
+
+
+@pytest.mark.parametrize("n,npart", [(256, 32), (512, 32)])
+def test_large_grid_fp32_step_matches_fp64(n, npart):
+    # the FP32 passes at these sizes keep each column's DFTs in registers (four-step); the
+    # FP64 sim runs the LDS Stockham passes.  Same state in, one step: potential and update
+    # agree to FP32 accuracy.
+    pos = uniform_positions(n, npart ** 3, seed=n)
+    vel = np.zeros_like(pos)
+    sims = [_sim(n, npart, 32), _sim(n, npart, 64)]
+    for s in sims:
+        s.set_state(pos, vel)
+        s.step(0.05, 0.01)
+    phi32, phi64 = (s.potential(np.float64) for s in sims)
+    (x32, v32), (x64, v64) = (s.get_state(np.float64) for s in sims)
+    e_phi, e_v = rel(phi32, phi64), rel(v32, v64)
+    print(f"N={n}: phi {e_phi:.2e} accel {e_v:.2e}")
+    assert e_phi < 1e-5 and e_v < 1e-5
+    assert np.abs(periodic_diff(x32, x64, n)).max() < 1e-4
+    for s in sims:
+        s.close()
+
+
+def test_fp32_step_256_matches_oracle():
+    n, npart = 256, 32
+    pos = uniform_positions(n, npart ** 3, seed=5)
+    vel = np.zeros_like(pos)
+    sim = _sim(n, npart, 32)
+    sim.set_state(pos, vel)
+    x0, v0 = sim.get_state(np.float32)
+    p = O.OracleParams(n_cells=n, n_parts=npart, omega_m0=0.3, omega_lambda0=0.7, h0=0.7)
+    rho_ref = O.density(x0, p.mass, n)
+    a, da = 0.05, 0.01
+    _, v1r, phi_ref = O.advance_time(rho_ref, x0, v0, O.fourier_grid(n), a, da, p, return_phi=True)
+    sim.step(a, da)
+    e_phi = rel(sim.potential(np.float64), phi_ref)
+    e_acc = rel(sim.get_state(np.float64)[1] - v0, v1r - v0)
+    print(f"N=256 fp32 vs oracle: phi {e_phi:.2e} accel {e_acc:.2e}")
+    assert e_phi < 1e-5 and e_acc < 1e-5
+    sim.close()
