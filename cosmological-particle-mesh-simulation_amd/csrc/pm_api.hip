@@ -402,8 +402,17 @@ struct pm_sim {
 
 namespace {
 
+// bins are allocated for the candidate tile shape with the most tiles, so that the tile shape
+// can change (autotuning) without reallocation
+TileGeom max_tiles(int n, int prec) {
+  TileGeom m = make_tiles(n, prec);
+  for (const TileGeom& g : tile_candidates(n, prec))
+    if (g.ntiles > m.ntiles) m = g;
+  return m;
+}
+
 size_t bins_bytes(int n, int np_side, int prec) {
-  const TileGeom g = make_tiles(n, prec);
+  const TileGeom g = max_tiles(n, prec);
   const size_t np = (size_t)np_side * np_side * np_side;
   return 4 * (3 * 2 * (size_t)g.ntiles + 1 + (size_t)scan_blocks(g)) + 4 * 8 * np;
 }
@@ -616,9 +625,109 @@ template <typename R> int sim_run_graph(pm_sim* s, double& a, double da, int n_s
   return PM_OK;
 }
 
+void sim_drop_graph(pm_sim* s) {
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  s->graph = nullptr;
+}
+
+// use tile shape g from now on (the captured graph holds the old shape's launches)
+int sim_use_tiles(pm_sim* s, const TileGeom& g) {
+  if (!g.ntiles) return fail(PM_E_UNSUPPORTED, "tile shape does not divide N_CELLS=%d", s->L.n);
+  PM_HIP(hipStreamSynchronize(s->stream));
+  if (g.tx != s->G.tx || g.ty != s->G.ty || g.tz != s->G.tz) {
+    s->G = g;
+    s->bins_valid = false;
+    s->rho_valid = false;
+    sim_drop_graph(s);
+  }
+  return PM_OK;
+}
+
+// Autotuner: times the particle stages (deposit, gather with zero kick and drift, which leaves
+// the state unchanged, and the binning that follows) for every tile shape that divides the
+// grid, on the sim's own particles, and keeps the fastest.
+template <typename R> int sim_autotune_impl(pm_sim* s, int reps, double* best_ms) {
+  const int n = s->L.n;
+  const size_t nn = (size_t)n;
+  // the no-op gather reads the potential buffer: give it zeros (finite forces times zero)
+  PM_HIP(hipMemsetAsync(s->H, 0, nn * nn * nn * s->prec, s->stream));
+  s->phi_valid = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PM_HIP(hipEventCreate(&e0));
+  PM_HIP(hipEventCreate(&e1));
+  double best = 1e300;
+  TileGeom bg = s->G;
+  int rc = PM_OK;
+  for (const TileGeom& g : tile_candidates(n, s->prec)) {
+    s->G = g;
+    if ((rc = sim_bin_impl<R>(s))) break;
+    float ms = 0.f;
+    hipError_t e = hipSuccess;
+    for (int r = -1; r < reps && e == hipSuccess; ++r) {  // r = -1: warm-up
+      if (r == 0) e = hipEventRecord(e0, s->stream);
+      if (!e) e = deposit_tiles<R>((const R*)s->s, (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream);
+      if (!e) e = gkd_tiles<R>((R*)s->s, (R*)s->v, (const R*)s->H, 0.0, 0.0, s->L, s->G, s->B, StepTable{}, s->stream);
+      if (!e) e = bins_scan(s->G, s->B, s->stream);
+      if (!e) e = bins_scatter<R>((const R*)s->s, s->L, s->G, s->B, s->stream);
+    }
+    if (!e) e = hipEventRecord(e1, s->stream);
+    if (!e) e = hipEventSynchronize(e1);
+    if (!e) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e) {
+      rc = fail(PM_E_HIP, "autotune timing failed: %s", hipGetErrorString(e));
+      break;
+    }
+    const double t = ms / std::max(reps, 1);
+    if (t < best) {
+      best = t;
+      bg = g;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s->G = bg;
+  s->bins_valid = false;
+  s->rho_valid = false;
+  sim_drop_graph(s);
+  if (rc) return rc;
+  if (best_ms) *best_ms = best;
+  return sim_deposit_impl<R>(s);
+}
+
 }  // namespace
 
 extern "C" {
+
+PM_API int pm_sim_set_tiles(pm_sim* s, int tx, int ty, int tz) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  if ((tx || ty || tz) && !tile_shape_supported(tx, ty, tz))
+    return fail(PM_E_UNSUPPORTED, "tile shape %dx%dx%d is not instantiated", tx, ty, tz);
+  if (s->ens && !(tx == 16 && ty == 16 && tz == 16) && (tx || ty || tz))
+    return fail(PM_E_STATE, "ensemble members keep 16^3 tiles");
+  return sim_use_tiles(s, make_tiles(s->L.n, s->prec, tx, ty, tz));
+}
+
+PM_API int pm_sim_get_tiles(pm_sim* s, int* tx, int* ty, int* tz) {
+  if (!s || !tx || !ty || !tz) return fail(PM_E_INVALID, "null argument");
+  *tx = s->G.tx;
+  *ty = s->G.ty;
+  *tz = s->G.tz;
+  return PM_OK;
+}
+
+PM_API int pm_sim_autotune(pm_sim* s, int reps, int* tx, int* ty, int* tz, double* best_ms) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  if (!s->has_state) return fail(PM_E_STATE, "state was never set");
+  if (s->ens) return fail(PM_E_STATE, "ensemble members keep 16^3 tiles");
+  int rc = s->prec == 4 ? sim_autotune_impl<float>(s, std::max(reps, 1), best_ms)
+                        : sim_autotune_impl<double>(s, std::max(reps, 1), best_ms);
+  if (rc) return rc;
+  PM_HIP(hipStreamSynchronize(s->stream));
+  if (tx) *tx = s->G.tx;
+  if (ty) *ty = s->G.ty;
+  if (tz) *tz = s->G.tz;
+  return PM_OK;
+}
 
 PM_API int pm_sim_footprint(int n_cells, int n_parts_side, int precision, size_t* bytes) {
   if (!bytes || n_cells <= 0 || n_parts_side <= 0 || (precision != 32 && precision != 64))
@@ -661,12 +770,13 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
       hipMalloc(&s->Q, nn * nn * 2 * pr) != hipSuccess)
     return cleanup(fail(PM_E_NOMEM, "device allocation of %zu bytes failed", sim_bytes(n_cells, n_parts_side, s->prec)));
   s->G = make_tiles(n_cells, s->prec);
-  const size_t m = 2 * (size_t)s->G.ntiles;
+  const TileGeom gmax = max_tiles(n_cells, s->prec);
+  const size_t m = 2 * (size_t)gmax.ntiles;
   if (8 * (size_t)s->L.np >= (size_t)UINT32_MAX) return cleanup(fail(PM_E_INVALID, "too many particles (N_PARTS > 812)"));
   s->B.capacity = (uint32_t)(8 * (size_t)s->L.np);
   if (hipMalloc(&s->B.counts, m * 4) != hipSuccess || hipMalloc(&s->B.offsets, (m + 1) * 4) != hipSuccess ||
       hipMalloc(&s->B.cursors, m * 4) != hipSuccess || hipMalloc(&s->B.entries, 8 * (size_t)s->L.np * 4) != hipSuccess ||
-      hipMalloc(&s->B.block_sums, (size_t)scan_blocks(s->G) * 4) != hipSuccess)
+      hipMalloc(&s->B.block_sums, (size_t)scan_blocks(gmax) * 4) != hipSuccess)
     return cleanup(fail(PM_E_NOMEM, "device allocation of the tile bins failed"));
   // every entry holds a valid slot from the start, counters start at zero
   if (hipMemset(s->B.entries, 0, 8 * (size_t)s->L.np * 4) != hipSuccess ||
@@ -1289,6 +1399,8 @@ PM_API int pm_ensemble_add(pm_ensemble* e, pm_sim* s) {
   if (s->ens == e) return fail(PM_E_INVALID, "the sim is already a member");
   if (s->ens) return fail(PM_E_STATE, "the sim is a member of another ensemble");
   if ((int)e->members.size() >= kBatchMax) return fail(PM_E_UNSUPPORTED, "an ensemble holds at most %d sims", kBatchMax);
+  if (s->G.tx != 16 || s->G.ty != 16 || s->G.tz != 16)  // members share the 16^3 tile kernels
+    if (int rc = sim_use_tiles(s, make_tiles(s->L.n, s->prec, 16, 16, 16))) return rc;
   if (int rc = ens_check_member(s)) return rc;
   PM_HIP(hipStreamSynchronize(e->stream));
   e->members.push_back(s);

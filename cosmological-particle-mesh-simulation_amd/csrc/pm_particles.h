@@ -1,6 +1,8 @@
 // Particle-side kernels of the PM step: state conversion, CIC deposit, fused gather+kick+drift,
 // and the reference-layout kernels behind the stateless C-ABI entry points.
 #pragma once
+#include <vector>
+
 #include "pm_common.h"
 #include "pm_fft.h"
 
@@ -91,7 +93,13 @@ struct Bins {
   uint32_t capacity = 0;           // entries allocated (8*N_p)
 };
 
-TileGeom make_tiles(int n, int prec_bytes);
+// Tile shapes (x, y, z cells) the tile kernels are instantiated for; the autotuner picks
+// among those that divide the grid.
+#define PM_TILE_SHAPES(X) X(16, 16, 16) X(32, 16, 8) X(32, 8, 8) X(16, 16, 8) X(64, 8, 8) X(8, 8, 8)
+bool tile_shape_supported(int tx, int ty, int tz);
+// tiles of shape (tx, ty, tz) on an n^3 grid (all 0: the default 16^3); ntiles = 0 if unusable
+TileGeom make_tiles(int n, int prec_bytes, int tx = 0, int ty = 0, int tz = 0);
+std::vector<TileGeom> tile_candidates(int n, int prec_bytes);
 int scan_blocks(const TileGeom& g);
 
 // ---- resident-state kernels (slot order, displacement storage) -------------------------------

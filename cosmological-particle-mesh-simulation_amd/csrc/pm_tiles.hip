@@ -14,6 +14,8 @@
 // The gather reads the own entries with the tile's potential (plus a -1/+2 stencil margin)
 // staged in LDS.  Counts come from a wave-aggregated atomic per distinct key (one atomic per
 // distinct tile in a wavefront), and the offsets from a three-kernel exclusive scan.
+#include <vector>
+
 #include "pm_batch.h"
 
 namespace pm {
@@ -385,11 +387,10 @@ __device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, c
                                          uint32_t* __restrict__ counts, uint32_t capacity, int tile, R* box,
                                          uint32_t* hist) {
   constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3;
-  static_assert(BX <= 64, "a box row must fit in one wavefront");
-  constexpr int RPW = 64 / BX;                      // box rows per wavefront load
   constexpr int NW = kT / 64;                       // wavefronts per block
-  constexpr int ROWS = BY * BZ;
-  constexpr int ITERS = (ROWS + RPW * NW - 1) / (RPW * NW);
+  constexpr int PLANE = BX * BY;                    // values per box plane
+  constexpr int PL = (PLANE + 63) / 64;             // wavefront loads per plane
+  constexpr int ZIT = (BZ + NW - 1) / NW;           // planes per wavefront
   constexpr int E = 2;
   // hist: new-tile counts, offset (-1..2)^3 from this tile x {own, halo}
   const uint32_t end = min(offsets[2 * tile + 1], capacity);
@@ -425,27 +426,39 @@ __device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, c
       }
   };
   load_entries(beg);
-  // potential of the tile and its -1/+2 stencil margin (periodic), one box row per lane group
+  // potential of the tile and its -1/+2 stencil margin (periodic).  Each wavefront loads whole
+  // z planes of the box (BY x BX values in PL loads of 64 lanes): a lane's in-plane offsets are
+  // computed once, and a plane adds one wave-uniform base.
   {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int rl = lane / BX, x = lane - rl * BX;
-    const bool lane_on = rl < RPW;
-    const int gx = fold(o0[0] - 1 + x, n);
-    R tmp[ITERS];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t off[PL];
 #pragma unroll
-    for (int it = 0; it < ITERS; ++it) {
-      const int row = RPW * (wv + NW * it) + rl;
-      if (lane_on && row < ROWS) {
-        const int by = row % BY, bz = row / BY;
-        const int gy = fold(o0[1] - 1 + by, n), gz = fold(o0[2] - 1 + bz, n);
-        tmp[it] = phi[((size_t)gz * nn + gy) * nn + gx];
+    for (int k = 0; k < PL; ++k) {
+      const int e = k * 64 + lane;
+      const int by = e / BX, x = e - by * BX;
+      off[k] = e < PLANE ? (uint32_t)(fold(o0[1] - 1 + by, n) * n + fold(o0[0] - 1 + x, n)) : 0u;
+    }
+    R tmp[ZIT][PL];
+#pragma unroll
+    for (int it = 0; it < ZIT; ++it) {
+      const int bz = wv + NW * it;
+      if (bz < BZ) {
+        const R* plane = phi + (size_t)fold(o0[2] - 1 + bz, n) * nn * nn;
+#pragma unroll
+        for (int k = 0; k < PL; ++k)
+          if (k * 64 + lane < PLANE) tmp[it][k] = plane[off[k]];
       }
     }
     load_state();
 #pragma unroll
-    for (int it = 0; it < ITERS; ++it) {
-      const int row = RPW * (wv + NW * it) + rl;
-      if (lane_on && row < ROWS) box[row * BX + x] = tmp[it];
+    for (int it = 0; it < ZIT; ++it) {
+      const int bz = wv + NW * it;
+      if (bz < BZ) {
+#pragma unroll
+        for (int k = 0; k < PL; ++k)
+          if (k * 64 + lane < PLANE) box[bz * PLANE + k * 64 + lane] = tmp[it][k];
+      }
     }
   }
   __syncthreads();
@@ -511,24 +524,33 @@ __device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, c
         }
         tile_of<TX, TY, TZ>(cn, G, tn, up);
         if (LOCAL_HIST) {
+          // new home tile relative to this one, folded into [-1, 2] per axis; when it lies in
+          // [-1, 1] every corner target (home + 0/1 per axis) is inside the 4^3 neighbourhood
           const int tc[3] = {tix, tiy, tiz};
           const int ntc[3] = {G.ntx, G.nty, G.ntz};
-#pragma unroll 1
-          for (int o = 0; o < 8; ++o) {
-            if (!target_active(up, o)) continue;
-            const int oo[3] = {o & 1, (o >> 1) & 1, o >> 2};
-            int rr[3];
-            bool near = true;
+          int rb[3];
+          bool near = true;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              int dlt = tn[a] + oo[a] - tc[a];  // periodic difference folded into [-1, 2]
-              if (dlt > 2) dlt -= ntc[a];
-              if (dlt < -1) dlt += ntc[a];
-              rr[a] = dlt + 1;
-              near = near && dlt >= -1 && dlt <= 2;
+          for (int a = 0; a < 3; ++a) {
+            int dlt = tn[a] - tc[a];
+            if (dlt > 2) dlt -= ntc[a];
+            if (dlt < -1) dlt += ntc[a];
+            rb[a] = dlt + 1;
+            near = near && dlt >= -1 && dlt <= 1;
+          }
+          if (near) {
+            const int h0 = 2 * ((rb[2] * 4 + rb[1]) * 4 + rb[0]);
+            atomicAdd(&hist[h0], 1u);
+            if (up[0] | up[1] | up[2]) {
+#pragma unroll
+              for (int o = 1; o < 8; ++o)
+                if (target_active(up, o))
+                  atomicAdd(&hist[h0 + 2 * (((o >> 2) * 4 + ((o >> 1) & 1)) * 4 + (o & 1)) + 1], 1u);
             }
-            if (near) atomicAdd(&hist[2 * ((rr[2] * 4 + rr[1]) * 4 + rr[0]) + (o != 0)], 1u);
-            else atomicAdd(&counts[target_key(tn, o, G)], 1u);  // moved beyond the neighbourhood
+          } else {
+#pragma unroll 1
+            for (int o = 0; o < 8; ++o)  // moved beyond the neighbourhood: global counters
+              if (target_active(up, o)) atomicAdd(&counts[target_key(tn, o, G)], 1u);
           }
         }
       }
@@ -646,33 +668,62 @@ hipError_t step_advance(int* step, hipStream_t st) {
   return hipGetLastError();
 }
 
-TileGeom make_tiles(int n, int prec_bytes) {
-  auto pick = [n](int pref) {
-    int t = pref;
-    while (t > 1 && n % t) t /= 2;
-    return t;
-  };
-  TileGeom g;
-  (void)prec_bytes;  // 16^3 tiles in both precisions (FP64 LDS accumulators in the deposit)
-  g.tx = pick(16);
-  g.ty = pick(16);
-  g.tz = pick(16);
-  if (g.ty < 8 || g.tz < 8 || g.tx < 8) g.tx = g.ty = g.tz = 0;  // not a supported shape
-  g.ntx = g.tx ? n / g.tx : 0;
-  g.nty = g.ty ? n / g.ty : 0;
-  g.ntz = g.tz ? n / g.tz : 0;
+bool tile_shape_supported(int tx, int ty, int tz) {
+#define PM_SHAPE(X, Y, Z) \
+  if (tx == X && ty == Y && tz == Z) return true;
+  PM_TILE_SHAPES(PM_SHAPE)
+#undef PM_SHAPE
+  return false;
+}
+
+TileGeom make_tiles(int n, int prec_bytes, int tx, int ty, int tz) {
+  TileGeom g{};
+  (void)prec_bytes;  // the same shapes in both precisions (FP64 LDS accumulators in the deposit)
+  if (tx == 0 && ty == 0 && tz == 0) {  // default: 16^3 (8^3 on the smallest grids)
+    auto pick = [n](int pref) {
+      int t = pref;
+      while (t > 1 && n % t) t /= 2;
+      return t;
+    };
+    tx = pick(16);
+    ty = pick(16);
+    tz = pick(16);
+  }
+  if (!tile_shape_supported(tx, ty, tz) || n % tx || n % ty || n % tz) return g;  // not usable
+  g.tx = tx;
+  g.ty = ty;
+  g.tz = tz;
+  g.ntx = n / tx;
+  g.nty = n / ty;
+  g.ntz = n / tz;
   g.ntiles = g.ntx * g.nty * g.ntz;
   return g;
+}
+
+std::vector<TileGeom> tile_candidates(int n, int prec_bytes) {
+  std::vector<TileGeom> out;
+#define PM_SHAPE(X, Y, Z)                                 \
+  {                                                       \
+    const TileGeom g = make_tiles(n, prec_bytes, X, Y, Z); \
+    if (g.ntiles) out.push_back(g);                       \
+  }
+  PM_TILE_SHAPES(PM_SHAPE)
+#undef PM_SHAPE
+  return out;
 }
 
 int scan_blocks(const TileGeom& g) { return (int)((2L * g.ntiles + kScanChunk - 1) / kScanChunk); }
 
 // tile shapes the kernels are instantiated for; make_tiles only produces these
-#define PM_TILE_DISPATCH(G, CALL)                                        \
-  do {                                                                   \
-    if ((G).tx == 16 && (G).ty == 16 && (G).tz == 16) CALL(16, 16, 16);      \
-    else if ((G).tx == 8 && (G).ty == 8 && (G).tz == 8) CALL(8, 8, 8);   \
-    else return hipErrorInvalidValue;                                    \
+#define PM_TILE_DISPATCH(G, CALL)                                                                   \
+  do {                                                                                              \
+    if ((G).tx == 16 && (G).ty == 16 && (G).tz == 16) CALL(16, 16, 16);                             \
+    else if ((G).tx == 32 && (G).ty == 16 && (G).tz == 8) CALL(32, 16, 8);                          \
+    else if ((G).tx == 32 && (G).ty == 8 && (G).tz == 8) CALL(32, 8, 8);                            \
+    else if ((G).tx == 16 && (G).ty == 16 && (G).tz == 8) CALL(16, 16, 8);                          \
+    else if ((G).tx == 64 && (G).ty == 8 && (G).tz == 8) CALL(64, 8, 8);                            \
+    else if ((G).tx == 8 && (G).ty == 8 && (G).tz == 8) CALL(8, 8, 8);                              \
+    else return hipErrorInvalidValue;                                                               \
   } while (0)
 
 template <typename R>

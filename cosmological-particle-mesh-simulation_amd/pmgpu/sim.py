@@ -61,6 +61,25 @@ class PMSim:
         self._green_ref = fgrid_device
         _lib.call("pm_sim_set_green", self._h, None if fgrid_device is None else fgrid_device.ptr)
 
+    # -- tile shape -------------------------------------------------------------------------
+    TILE_SHAPES = ((16, 16, 16), (32, 16, 8), (32, 8, 8), (16, 16, 8), (64, 8, 8), (8, 8, 8))
+
+    def set_tiles(self, tx: int = 0, ty: int = 0, tz: int = 0):
+        """Tile shape of the deposit / gather kernels (0, 0, 0: the default)."""
+        _lib.call("pm_sim_set_tiles", self._h, int(tx), int(ty), int(tz))
+
+    def tiles(self) -> tuple[int, int, int]:
+        t = [C.c_int() for _ in range(3)]
+        _lib.call("pm_sim_get_tiles", self._h, *[C.byref(x) for x in t])
+        return tuple(x.value for x in t)
+
+    def autotune(self, reps: int = 3):
+        """Pick the fastest tile shape on the current particles; returns (shape, ms per rep)."""
+        t = [C.c_int() for _ in range(3)]
+        ms = C.c_double()
+        _lib.call("pm_sim_autotune", self._h, int(reps), *[C.byref(x) for x in t], C.byref(ms))
+        return tuple(x.value for x in t), ms.value
+
     # -- state ------------------------------------------------------------------------------
     def set_state(self, positions, velocities):
         pos = np.ascontiguousarray(positions)

@@ -131,6 +131,16 @@ PM_API int pm_sim_get_timing(pm_sim* sim, double* stage_ms, int64_t* stage_calls
 PM_API int pm_sim_sync(pm_sim* sim);
 /* bytes of device memory held by the sim                                                     */
 PM_API int pm_sim_footprint(int n_cells, int n_parts_side, int precision, size_t* bytes);
+/* Eulerian tile shape of the deposit / gather kernels (cells per tile along x, y, z; all 0 =
+ * the default 16^3).  Instantiated shapes: 16x16x16, 32x16x8, 32x8x8, 16x16x8, 64x8x8, 8x8x8;
+ * each must divide N_CELLS.  No reference counterpart (the reference scatters per particle,
+ * density.py:17).                                                                            */
+PM_API int pm_sim_set_tiles(pm_sim* sim, int tx, int ty, int tz);
+PM_API int pm_sim_get_tiles(pm_sim* sim, int* tx, int* ty, int* tz);
+/* Autotuner: times deposit + gather + binning on the sim's own particles for every usable tile
+ * shape (reps timed repetitions each, the state is left unchanged) and keeps the fastest;
+ * returns the shape and its time per repetition in ms.                                     */
+PM_API int pm_sim_autotune(pm_sim* sim, int reps, int* tx, int* ty, int* tz, double* best_ms);
 
 /* ------------------------------------------------ the step's two ends: ICs and the spectrum */
 /* gaussian_random_field.py:32-50 draws the field's normal deviates from an unseeded parallel

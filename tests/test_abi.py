@@ -40,10 +40,11 @@ def test_pure_host_entry_points():
     assert abs(lib.pm_f(0.5, 0.3, 0.7, 0.0) - 1 / (0.5 * (0.3 / 0.125 + 0.7) ** 0.5)) < 1e-14
     b = ctypes.c_size_t()
     assert lib.pm_sim_footprint(512, 256, 32, ctypes.byref(b)) == 0
-    # particles + rho + half spectrum + Nyquist plane + tile bins (16^3 tiles, 8 entries per
-    # particle, counts/offsets/cursors and a 16-block scan spine)
-    ntiles = (512 // 16) ** 3
-    bins = 4 * (3 * 2 * ntiles + 1 + 16) + 4 * 8 * 256 ** 3
+    # particles + rho + half spectrum + Nyquist plane + tile bins (8 entries per particle,
+    # counts/offsets/cursors and scan spine sized for the tile shape with the most tiles,
+    # 8^3: 2 * 64^3 counters in chunks of 4096)
+    ntiles = (512 // 8) ** 3
+    bins = 4 * (3 * 2 * ntiles + 1 + 2 * ntiles // 4096) + 4 * 8 * 256 ** 3
     assert b.value == 6 * 256 ** 3 * 4 + 2 * 512 ** 3 * 4 + 512 * 512 * 2 * 4 + bins
     assert lib.pm_sim_footprint(512, 256, 16, ctypes.byref(b)) != 0
     assert lib.pm_supported_grid(512) == 1 and lib.pm_supported_grid(100) == 0

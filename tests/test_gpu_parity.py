@@ -331,3 +331,55 @@ def test_fp32_step_256_matches_oracle():
     print(f"N=256 fp32 vs oracle: phi {e_phi:.2e} accel {e_acc:.2e}")
     assert e_phi < 1e-5 and e_acc < 1e-5
     sim.close()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 16, 8), (32, 8, 8), (16, 16, 8), (64, 8, 8),
+                                   (8, 8, 8)])
+def test_every_tile_shape_matches_oracle(shape):
+    n, npart = 64, 32
+    pos, vel = make_ics(n, npart, seed=29)
+    sim = _sim(n, npart, 32)
+    sim.set_tiles(*shape)
+    assert sim.tiles() == shape
+    sim.set_state(pos, vel)
+    x0, v0 = sim.get_state(np.float32)
+    p = O.OracleParams(n_cells=n, n_parts=npart, omega_m0=0.3, omega_lambda0=0.7, h0=0.7)
+    rho_ref = O.density(x0, p.mass, n)
+    assert rel(sim.density(np.float64), rho_ref) < 1e-6
+    a, da = 0.02, 0.0098
+    _, v1r, _ = O.advance_time(rho_ref, x0, v0, O.fourier_grid(n), a, da, p, return_phi=True)
+    sim.run(a, da, 1)
+    sim.run(a + da, da, 9)                      # a captured graph with this shape, replayed
+    x1, v1 = sim.get_state(np.float64)
+    ref = _sim(n, npart, 32)                    # default tiles, same ten steps
+    ref.set_state(pos, vel)
+    ref.run(a, da, 1)
+    ref.run(a + da, da, 9)
+    xr, vr = ref.get_state(np.float64)
+    assert np.abs(periodic_diff(x1, xr, n)).max() < 1e-4 and rel(v1, vr) < 1e-5
+    with pytest.raises(RuntimeError, match="not instantiated"):
+        sim.set_tiles(16, 8, 4)
+    sim.close()
+    ref.close()
+
+
+def test_autotune_keeps_state_and_results():
+    n, npart = 128, 64
+    pos, vel = make_ics(n, npart, seed=37)
+    sim, ref = _sim(n, npart, 32), _sim(n, npart, 32)
+    for s in (sim, ref):
+        s.set_state(pos, vel)
+    x0, v0 = sim.get_state()
+    shape, ms = sim.autotune(reps=2)
+    print(f"autotune N=128: {shape} {ms:.3f} ms")
+    assert shape in sim.TILE_SHAPES and sim.tiles() == shape and ms > 0
+    x1, v1 = sim.get_state()
+    np.testing.assert_array_equal(x0, x1)
+    np.testing.assert_array_equal(v0, v1)
+    for s in (sim, ref):
+        s.run(0.02, 0.01, 12)
+    xa, va = sim.get_state(np.float64)
+    xb, vb = ref.get_state(np.float64)
+    assert np.abs(periodic_diff(xa, xb, n)).max() < 1e-4 and rel(va, vb) < 1e-5
+    sim.close()
+    ref.close()
