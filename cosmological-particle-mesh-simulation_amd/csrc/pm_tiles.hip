@@ -14,6 +14,7 @@
 // The gather reads the own entries with the tile's potential (plus a -1/+2 stencil margin)
 // staged in LDS.  Counts come from a wave-aggregated atomic per distinct key (one atomic per
 // distinct tile in a wavefront), and the offsets from a three-kernel exclusive scan.
+#include <algorithm>
 #include <vector>
 
 #include "pm_batch.h"
@@ -380,21 +381,39 @@ __global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R
 // ------------------------------------------------------------------------------------------
 template <int TX, int TY, int TZ> constexpr int gkd_box_cells() { return (TX + 3) * (TY + 3) * (TZ + 3); }
 
+constexpr int kGkdE = 2;  // particles per thread and batch in the gather
+
+// own-entry range [beg, end) of a tile, and a thread's first batch of its slot entries
+__device__ __forceinline__ void tile_range(const uint32_t* __restrict__ offsets, uint32_t capacity, int tile,
+                                           uint32_t& beg, uint32_t& end) {
+  end = min(offsets[2 * tile + 1], capacity);
+  beg = min(offsets[2 * tile], end);
+}
+__device__ __forceinline__ void first_entries(const uint32_t* __restrict__ entries, uint32_t beg, uint32_t end,
+                                              long np, uint32_t (&sl)[kGkdE]) {
+#pragma unroll
+  for (int u = 0; u < kGkdE; ++u) {
+    const uint32_t e = beg + u * kT + threadIdx.x;
+    sl[u] = e < end ? min(entries[e], (uint32_t)(np - 1)) : kNone;
+  }
+}
+
+// Gather + kick + drift of one tile's own particles.  [beg, end) is the tile's own-entry range
+// and sl0 the thread's first batch of entries (fetched by the caller before the tile's
+// potential box, so the two dependent load chains overlap).
 template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
-__device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi, R kick,
+__device__ __forceinline__ void gkd_tile(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi, R kick,
                                          R drift, const Lattice& L, const TileGeom& G,
-                                         const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ entries,
-                                         uint32_t* __restrict__ counts, uint32_t capacity, int tile, R* box,
+                                         const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
+                                         int tile, uint32_t beg, uint32_t end, const uint32_t (&sl0)[kGkdE], R* box,
                                          uint32_t* hist) {
   constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3;
   constexpr int NW = kT / 64;                       // wavefronts per block
   constexpr int PLANE = BX * BY;                    // values per box plane
   constexpr int PL = (PLANE + 63) / 64;             // wavefront loads per plane
   constexpr int ZIT = (BZ + NW - 1) / NW;           // planes per wavefront
-  constexpr int E = 2;
+  constexpr int E = kGkdE;
   // hist: new-tile counts, offset (-1..2)^3 from this tile x {own, halo}
-  const uint32_t end = min(offsets[2 * tile + 1], capacity);
-  const uint32_t beg = min(offsets[2 * tile], end);
   if (beg == end) return;  // empty tile: the whole block leaves together
   const int tix = tile % G.ntx, tiy = (tile / G.ntx) % G.nty, tiz = tile / (G.ntx * G.nty);
   const int o0[3] = {tix * TX, tiy * TY, tiz * TZ};
@@ -408,6 +427,8 @@ __device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, c
   // The first batch of entries is fetched before the potential box, the batch's particle
   // state together with the box, so the two dependent load chains overlap.
   uint32_t sl[E];
+#pragma unroll
+  for (int u = 0; u < E; ++u) sl[u] = sl0[u];
   R sv[E][3], vv[E][3];
   auto load_entries = [&](uint32_t base) {
 #pragma unroll
@@ -425,7 +446,6 @@ __device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, c
         vv[u][a] = sl[u] != kNone ? v[a * np + sl[u]] : R(0);
       }
   };
-  load_entries(beg);
   // potential of the tile and its -1/+2 stencil margin (periodic).  Each wavefront loads whole
   // z planes of the box (BY x BX values in PL loads of 64 lanes): a lane's in-plane offsets are
   // computed once, and a plane adds one wave-uniform base.
@@ -575,6 +595,18 @@ __device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, c
       }
     }
   }
+}
+
+template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
+__device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi, R kick,
+                                         R drift, const Lattice& L, const TileGeom& G,
+                                         const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ entries,
+                                         uint32_t* __restrict__ counts, uint32_t capacity, int tile, R* box,
+                                         uint32_t* hist) {
+  uint32_t beg, end, sl[kGkdE];
+  tile_range(offsets, capacity, tile, beg, end);
+  first_entries(entries, beg, end, L.np, sl);
+  gkd_tile<R, TX, TY, TZ, LOCAL_HIST>(s, v, phi, kick, drift, L, G, entries, counts, tile, beg, end, sl, box, hist);
 }
 
 template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
