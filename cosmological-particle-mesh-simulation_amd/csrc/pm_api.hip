@@ -414,7 +414,7 @@ TileGeom max_tiles(int n, int prec) {
 size_t bins_bytes(int n, int np_side, int prec) {
   const TileGeom g = max_tiles(n, prec);
   const size_t np = (size_t)np_side * np_side * np_side;
-  return 4 * (3 * 2 * (size_t)g.ntiles + 1 + (size_t)scan_blocks(g)) + 4 * 8 * np;
+  return 4 * (3 * 2 * (size_t)g.ntiles + 1 + (size_t)scan_blocks(g) + 1) + 4 * 8 * np;
 }
 
 size_t sim_bytes(int n, int np_side, int prec) {
@@ -472,11 +472,11 @@ template <typename R> int sim_bin_impl(pm_sim* s) {
   return PM_OK;
 }
 
-template <typename R> int sim_deposit_impl(pm_sim* s) {
+template <typename R> int sim_deposit_impl(pm_sim* s, int* step = nullptr) {
   if (!s->bins_valid)
     if (int rc = sim_bin_impl<R>(s)) return rc;
   StageTimer t(s, ST_DEPOSIT);
-  PM_HIP(deposit_tiles<R>((const R*)s->s, (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream));
+  PM_HIP(deposit_tiles<R>((const R*)s->s, (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream, step));
   s->rho_valid = true;
   return PM_OK;
 }
@@ -558,8 +558,7 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
     PM_HIP(bins_scan(s->G, s->B, s->stream));
     PM_HIP(bins_scatter<R>((const R*)s->s, s->L, s->G, s->B, s->stream));
   }
-  if (int rc = sim_deposit_impl<R>(s)) return rc;
-  if (table) PM_HIP(step_advance(s->d_step, s->stream));
+  if (int rc = sim_deposit_impl<R>(s, table ? s->d_step : nullptr)) return rc;
   return PM_OK;
 }
 
@@ -776,10 +775,12 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
   s->B.capacity = (uint32_t)(8 * (size_t)s->L.np);
   if (hipMalloc(&s->B.counts, m * 4) != hipSuccess || hipMalloc(&s->B.offsets, (m + 1) * 4) != hipSuccess ||
       hipMalloc(&s->B.cursors, m * 4) != hipSuccess || hipMalloc(&s->B.entries, 8 * (size_t)s->L.np * 4) != hipSuccess ||
-      hipMalloc(&s->B.block_sums, (size_t)scan_blocks(gmax) * 4) != hipSuccess)
+      hipMalloc(&s->B.block_sums, ((size_t)scan_blocks(gmax) + 1) * 4) != hipSuccess)
     return cleanup(fail(PM_E_NOMEM, "device allocation of the tile bins failed"));
   // every entry holds a valid slot from the start, counters start at zero
+  s->B.done = s->B.block_sums + scan_blocks(gmax);
   if (hipMemset(s->B.entries, 0, 8 * (size_t)s->L.np * 4) != hipSuccess ||
+      hipMemset(s->B.done, 0, 4) != hipSuccess ||
       hipMemset(s->B.counts, 0, m * 4) != hipSuccess || hipMemset(s->B.cursors, 0, m * 4) != hipSuccess ||
       hipMemset(s->B.offsets, 0, (m + 1) * 4) != hipSuccess)
     return cleanup(fail(PM_E_HIP, "clearing the tile bins failed"));
@@ -1241,6 +1242,7 @@ void ens_describe(pm_ensemble* e) {
     m.cursors = s->B.cursors;
     m.entries = s->B.entries;
     m.block_sums = s->B.block_sums;
+    m.scan_done = s->B.done;
     m.scan_nb = scan_blocks(s->G);
     D.max_n = std::max(D.max_n, n);
     long blocks[BS_COUNT];
@@ -1313,9 +1315,8 @@ int ens_step(pm_ensemble* e) {
   }
   {
     EnsTimer t(e, ST_DEPOSIT);
-    PM_HIP(batch_deposit(d, e->total[BS_TILES], st));
+    PM_HIP(batch_deposit(d, e->total[BS_TILES], st));  // also advances the step counter
   }
-  PM_HIP(step_advance(e->d_step, st));
   return PM_OK;
 }
 

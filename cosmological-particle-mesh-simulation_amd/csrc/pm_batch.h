@@ -43,6 +43,7 @@ struct BatchMember {
   uint32_t* cursors;
   uint32_t* entries;
   uint32_t* block_sums;
+  uint32_t* scan_done;
   int scan_nb;  // scan chunks of this member
 };
 
@@ -52,7 +53,7 @@ struct BatchDesc {
   long off[BS_COUNT][kBatchMax + 1];  // block prefix per stage
   // per-step scalars: table[(step * n_members + member) * 3 + {0: Green, 1: kick, 2: drift}]
   const double* table;
-  const int* step;
+  int* step;  // advanced by the step's last kernel (the deposit)
   BatchMember m[kBatchMax];
 };
 

@@ -867,10 +867,17 @@ __global__ __launch_bounds__(kThreads) void k_batch_pencil(const BatchDesc* __re
   cpx<float>* lds = reinterpret_cast<cpx<float>*>(smem);
 #define PM_CASE(NN)                                                                              \
   case NN: {                                                                                     \
-    cpx<float>* tw = lds + NN * (pencil_w<float, NN>() + 1);                                     \
-    float* s2 = reinterpret_cast<float*>(tw + NN);                                               \
-    pencil_body<float, NN, MODE>(m.H, m.Q, MODE == 0 ? m.za : m.ya, MODE == 0 ? m.zb : m.yb, g,  \
-                                 m.tw, m.sin2, m.wdec, blk, lds, tw, s2, s2 + NN);               \
+    if constexpr (reg_transform<float, NN>()) {                                                  \
+      cpx<float>* tw = lds + xslice_elems<float, NN>();                                          \
+      float* s2 = reinterpret_cast<float*>(tw + NN);                                             \
+      pencil_reg_body<float, NN, MODE>(m.H, m.Q, MODE == 0 ? m.za : m.ya, MODE == 0 ? m.zb : m.yb, \
+                                       g, m.tw, m.sin2, m.wdec, blk, lds, tw, s2, s2 + NN);      \
+    } else {                                                                                     \
+      cpx<float>* tw = lds + NN * (pencil_w<float, NN>() + 1);                                   \
+      float* s2 = reinterpret_cast<float*>(tw + NN);                                             \
+      pencil_body<float, NN, MODE>(m.H, m.Q, MODE == 0 ? m.za : m.ya, MODE == 0 ? m.zb : m.yb, g, \
+                                   m.tw, m.sin2, m.wdec, blk, lds, tw, s2, s2 + NN);             \
+    }                                                                                            \
     break;                                                                                       \
   }
   PM_BATCH_CLASS_SWITCH(CLS, m.n, PM_CASE)
@@ -932,6 +939,8 @@ int batch_fft_lds(int n, int stage) {
   case NN:                                                                                           \
     if (stage == BS_R2C) return xpass_lds<float, NN, x_rb<NN>()>();                                  \
     if (stage == BS_C2R) return xpass_lds<float, NN, x_rb<NN>()>() + x_rb<NN>() * 8;                 \
+    if (reg_transform<float, NN>())                                                                  \
+      return (xslice_elems<float, NN>() + NN) * 8 + 2 * NN * 4;                                       \
     return pencil_lds<float, NN>();
     PM_FOR_EACH_BATCH_N(PM_CASE)
 #undef PM_CASE

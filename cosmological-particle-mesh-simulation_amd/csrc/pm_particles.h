@@ -90,6 +90,7 @@ struct Bins {
   uint32_t* cursors = nullptr;     // 2*ntiles
   uint32_t* entries = nullptr;     // 8*N_p
   uint32_t* block_sums = nullptr;  // scan spine
+  uint32_t* done = nullptr;        // ticket counter of the scan's last-block spine (zero between scans)
   uint32_t capacity = 0;           // entries allocated (8*N_p)
 };
 
@@ -114,15 +115,14 @@ hipError_t bins_count(const R* s, const Lattice& L, const TileGeom& G, const Bin
 hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st);
 template <typename R>
 hipError_t bins_scatter(const R* s, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
+// step (optional): a table-driven step's counter, advanced by the deposit as the step's last kernel
 template <typename R>
 hipError_t deposit_tiles(const R* s, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
-                         hipStream_t st);
+                         hipStream_t st, int* step = nullptr);
 // also counts the entries of the new positions into B.counts
 template <typename R>
 hipError_t gkd_tiles(R* s, R* v, const R* phi, double kick, double drift, const Lattice& L, const TileGeom& G,
                      const Bins& B, const StepTable& tab, hipStream_t st);
-// *step += 1 on the device (advances a captured step's scalar table)
-hipError_t step_advance(int* step, hipStream_t st);
 
 // ---- reference-layout kernels (absolute float32 positions, reference order) ------------------
 hipError_t ref_density(const float* pos, long np, int n, double mass, float* rho, hipStream_t st);

@@ -152,14 +152,31 @@ __device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const 
     cell_and_frac<R>(s, slot, L, c, d);
     tile_of<TX, TY, TZ>(c, G, t, up);
   }
+  // home tile (o = 0) of every particle: its segment bounds load alongside the cursor atomic
+  {
+    const uint32_t key = valid ? target_key(t, 0, G) : 0u;
+    uint32_t beg = 0, lim = 0;
+    if (valid) {
+      beg = offsets[key];
+      lim = offsets[key + 1];
+    }
+    const uint32_t rank = wave_add<true>(cursors, key, valid);
+    if (valid) {
+      const uint32_t pos = beg + rank;
+      if (pos < lim && pos < capacity) entries[pos] = (uint32_t)slot;
+    }
+  }
+  // halo entries (clouds reaching the next tile) are rare: only waves that have any
+  if (__ballot(valid && (up[0] | up[1] | up[2]))) {
 #pragma unroll 1
-  for (int o = 0; o < 8; ++o) {
-    const bool act = valid && target_active(up, o);
-    const uint32_t key = act ? target_key(t, o, G) : 0u;
-    const uint32_t rank = wave_add<true>(cursors, key, act);
-    if (act) {
-      const uint32_t pos = offsets[key] + rank;
-      if (pos < offsets[key + 1] && pos < capacity) entries[pos] = (uint32_t)slot;
+    for (int o = 1; o < 8; ++o) {
+      const bool act = valid && target_active(up, o);
+      const uint32_t key = act ? target_key(t, o, G) : 0u;
+      const uint32_t rank = wave_add<true>(cursors, key, act);
+      if (act) {
+        const uint32_t pos = offsets[key] + rank;
+        if (pos < offsets[key + 1] && pos < capacity) entries[pos] = (uint32_t)slot;
+      }
     }
   }
 }
@@ -206,12 +223,6 @@ __device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ co
   if (threadIdx.x == 0) bsum[blk] = total;
 }
 
-__global__ __launch_bounds__(kT) void k_scan_reduce(const uint32_t* __restrict__ counts, long m,
-                                                    uint32_t* __restrict__ bsum) {
-  __shared__ uint32_t sh[kT];
-  scan_reduce_body(counts, m, bsum, blockIdx.x, sh);
-}
-
 __device__ __forceinline__ void scan_spine_body(uint32_t* __restrict__ bsum, int nb, uint32_t* sh) {
   uint32_t carry = 0;
   for (int base = 0; base < nb; base += kT) {
@@ -224,9 +235,30 @@ __device__ __forceinline__ void scan_spine_body(uint32_t* __restrict__ bsum, int
   }
 }
 
-__global__ __launch_bounds__(kT) void k_scan_spine(uint32_t* __restrict__ bsum, int nb) {
+// Chunk sums, then the last block to finish scans them (the spine): each block publishes its
+// sum, fences, and takes a ticket; the block holding the last ticket fences again, scans the
+// sums in place and resets the ticket counter for the next scan.  No block waits on another.
+__device__ __forceinline__ void scan_reduce_spine_body(const uint32_t* __restrict__ counts, long m,
+                                                       uint32_t* __restrict__ bsum, int nb, uint32_t* done,
+                                                       long blk, uint32_t* sh, uint32_t* last) {
+  scan_reduce_body(counts, m, bsum, blk, sh);
+  if (threadIdx.x == 0) {
+    __threadfence();
+    *last = atomicAdd(done, 1u) == (uint32_t)(nb - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (*last) {
+    __threadfence();
+    scan_spine_body(bsum, nb, sh);
+    if (threadIdx.x == 0) *done = 0u;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_scan_reduce_spine(const uint32_t* __restrict__ counts, long m,
+                                                          uint32_t* __restrict__ bsum, int nb, uint32_t* done) {
   __shared__ uint32_t sh[kT];
-  scan_spine_body(bsum, nb, sh);
+  __shared__ uint32_t last;
+  scan_reduce_spine_body(counts, m, bsum, nb, done, blockIdx.x, sh, &last);
 }
 
 __device__ __forceinline__ void scan_apply_body(uint32_t* __restrict__ counts, long m,
@@ -359,13 +391,17 @@ __device__ __forceinline__ void deposit_body(const R* __restrict__ s, R* __restr
   }
 }
 
+// step: when set, the last kernel of a table-driven step advances the table's step counter
+// (no kernel of the step reads it after the gather)
 template <typename R, int TX, int TY, int TZ>
 __global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R* __restrict__ rho, R mass,
                                                       Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
-                                                      const uint32_t* __restrict__ entries, uint32_t capacity) {
+                                                      const uint32_t* __restrict__ entries, uint32_t capacity,
+                                                      int* step) {
   __shared__ double acc[deposit_lds_cells<TX, TY, TZ>()];
   deposit_body<R, TX, TY, TZ>(s, rho, mass, L, G, offsets, entries, capacity, xcd_tile(blockIdx.x, gridDim.x),
                               acc);
+  if (step && blockIdx.x == 0 && threadIdx.x == 0) *step += 1;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -657,6 +693,7 @@ __global__ __launch_bounds__(kT) void k_batch_deposit(const BatchDesc* __restric
   const BatchMember& m = D->m[k];
   deposit_body<float, kBT, kBT, kBT>(m.s, m.rho, m.mass, m.L, m.G, m.offsets, m.entries, m.capacity,
                                      (int)(b - D->off[BS_TILES][k]), acc);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *D->step += 1;  // the step's last kernel
 }
 
 __global__ __launch_bounds__(kT) void k_batch_scatter(const BatchDesc* __restrict__ D) {
@@ -669,16 +706,12 @@ __global__ __launch_bounds__(kT) void k_batch_scatter(const BatchDesc* __restric
 
 __global__ __launch_bounds__(kT) void k_batch_scan_reduce(const BatchDesc* __restrict__ D) {
   __shared__ uint32_t sh[kT];
+  __shared__ uint32_t last;
   const long b = blockIdx.x;
   const int k = batch_member(D, BS_SCAN, b);
   const BatchMember& m = D->m[k];
-  scan_reduce_body(m.counts, 2L * m.G.ntiles, m.block_sums, b - D->off[BS_SCAN][k], sh);
-}
-
-__global__ __launch_bounds__(kT) void k_batch_scan_spine(const BatchDesc* __restrict__ D) {
-  __shared__ uint32_t sh[kT];
-  const BatchMember& m = D->m[blockIdx.x];
-  scan_spine_body(m.block_sums, m.scan_nb, sh);
+  scan_reduce_spine_body(m.counts, 2L * m.G.ntiles, m.block_sums, m.scan_nb, m.scan_done, b - D->off[BS_SCAN][k],
+                         sh, &last);
 }
 
 __global__ __launch_bounds__(kT) void k_batch_scan_apply(const BatchDesc* __restrict__ D) {
@@ -689,16 +722,8 @@ __global__ __launch_bounds__(kT) void k_batch_scan_apply(const BatchDesc* __rest
   scan_apply_body(m.counts, 2L * m.G.ntiles, m.block_sums, m.offsets, m.cursors, b - D->off[BS_SCAN][k], sh);
 }
 
-__global__ void k_step_advance(int* step) {
-  if (threadIdx.x == 0) *step += 1;
-}
-
 }  // namespace
 
-hipError_t step_advance(int* step, hipStream_t st) {
-  k_step_advance<<<1, 64, 0, st>>>(step);
-  return hipGetLastError();
-}
 
 bool tile_shape_supported(int tx, int ty, int tz) {
 #define PM_SHAPE(X, Y, Z) \
@@ -769,8 +794,7 @@ hipError_t bins_count(const R* s, const Lattice& L, const TileGeom& G, const Bin
 hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st) {
   const long m = 2L * G.ntiles;
   const int nb = scan_blocks(G);
-  k_scan_reduce<<<nb, kT, 0, st>>>(B.counts, m, B.block_sums);
-  k_scan_spine<<<1, kT, 0, st>>>(B.block_sums, nb);
+  k_scan_reduce_spine<<<nb, kT, 0, st>>>(B.counts, m, B.block_sums, nb, B.done);
   k_scan_apply<<<nb, kT, 0, st>>>(B.counts, m, B.block_sums, B.offsets, B.cursors);
   return hipGetLastError();
 }
@@ -786,9 +810,9 @@ hipError_t bins_scatter(const R* s, const Lattice& L, const TileGeom& G, const B
 
 template <typename R>
 hipError_t deposit_tiles(const R* s, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
-                         hipStream_t st) {
+                         hipStream_t st, int* step) {
 #define PM_L(X, Y, Z) \
-  k_deposit_tiles<R, X, Y, Z><<<G.ntiles, kT, 0, st>>>(s, rho, R(mass), L, G, B.offsets, B.entries, B.capacity)
+  k_deposit_tiles<R, X, Y, Z><<<G.ntiles, kT, 0, st>>>(s, rho, R(mass), L, G, B.offsets, B.entries, B.capacity, step)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
   return hipGetLastError();
@@ -846,7 +870,6 @@ hipError_t batch_scan(const BatchDesc* d, long total, int n_members, hipStream_t
   if (total <= 0 || n_members <= 0) return hipSuccess;
   if (hipError_t e = batch_grid(total, g)) return e;
   k_batch_scan_reduce<<<g, kT, 0, st>>>(d);
-  k_batch_scan_spine<<<n_members, kT, 0, st>>>(d);
   k_batch_scan_apply<<<g, kT, 0, st>>>(d);
   return hipGetLastError();
 }
@@ -855,7 +878,7 @@ hipError_t batch_scan(const BatchDesc* d, long total, int n_members, hipStream_t
   template hipError_t bins_count<R>(const R*, const Lattice&, const TileGeom&, const Bins&, hipStream_t);    \
   template hipError_t bins_scatter<R>(const R*, const Lattice&, const TileGeom&, const Bins&, hipStream_t);  \
   template hipError_t deposit_tiles<R>(const R*, R*, double, const Lattice&, const TileGeom&, const Bins&,   \
-                                       hipStream_t);                                                         \
+                                       hipStream_t, int*);                                                   \
   template hipError_t gkd_tiles<R>(R*, R*, const R*, double, double, const Lattice&, const TileGeom&,        \
                                    const Bins&, const StepTable&, hipStream_t);
 PM_INST(float)
