@@ -225,6 +225,40 @@ def ensemble_bench(e, steps=None):
             "speedup_vs_one_at_a_time": t_solo / t_ens}
 
 
+# Largest single run: 512^3 particles on a 1024^3 grid, box 500 Mpc/h, seed 42, a 0.02 -> 1 in
+# 500 steps; device initial conditions (the host generator would need tens of GB of RAM).
+LARGEST = dict(WORKLOAD, n_cells=1024, n_parts=512, box=500.0, seed=42, steps=500)
+
+
+def largest_bench(w, steps):
+    """A bounded sample (`steps` of the 500) of the largest run, FP32, graph-replayed, timed
+    like the main workload."""
+    import initial_conditions as ic
+    from pmgpu.sim import PMSim
+    sim = PMSim(w["n_cells"], w["n_parts"], 32, omega_m0=w["om"], omega_lambda0=w["ol"],
+                omega_k0=w["ok"], h0=w["h"])
+    ic.zeldovich_device(sim, _ic_params(w))
+    tiles, _ = sim.autotune(2)
+    pos, vel = sim.get_state()
+    da = (w["a_end"] - w["a_init"]) / w["steps"]
+    sim.run(w["a_init"], da, 8)                       # capture + warm-up
+    sim.set_state(pos, vel)
+    sim.deposit()
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(w["a_init"], da, steps)
+    sim.sync()
+    dt = time.perf_counter() - t0
+    n, n_p = w["n_cells"], w["n_parts"] ** 3
+    sim.close()
+    ms = dt / steps * 1e3
+    return {"workload": "pm_step_512p_1024g: 512^3 particles, 1024^3 grid, box 500 Mpc/h, seed 42, "
+                        f"first {steps} of 500 steps from a = 0.02, device ICs",
+            "value": n_p * steps / dt, "unit": "particle-updates/s", "ms_per_step": ms,
+            "steps": steps, "tiles": list(tiles),
+            "step_roofline_frac": step_bytes(n, n_p, 32) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 # Serving stream: Poisson arrivals at 2 requests/s for 600 s, N_grid in {64..384} with weights
 # 0.3/0.3/0.2/0.15/0.05, N_parts = N_grid/2, box U(50, 400), STEPS U{50..500}, sequential seeds.
 SERVE = dict(rate=2.0, duration=600.0, seed=4, sizes=(64, 128, 192, 256, 384),
@@ -387,6 +421,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--largest-steps", type=int, default=20,
+                    help="steps of the 512^3 / 1024^3 run to time (0 skips it)")
     ap.add_argument("--serve-window", type=float, default=60.0,
                     help="seconds of the 600 s serving stream to replay (600 = all of it)")
     ap.add_argument("--ensemble-steps", type=int, default=None)
@@ -474,6 +510,11 @@ def main():
     out["fp32_over_plain_fp64"] = v32 / out["plain_fp64"]["value"]
     out["ensemble"] = None if args.no_ensemble else ensemble_bench(ENSEMBLE, args.ensemble_steps)
     out["serve"] = None if args.no_serve else serve_bench(SERVE, args.serve_window)
+    if args.largest_steps > 0:
+        log(f"largest run: {args.largest_steps} steps of 512^3 particles on 1024^3")
+        out["largest"] = largest_bench(LARGEST, args.largest_steps)
+    else:
+        out["largest"] = None
     if not args.no_cpu:
         log("CPU baseline (port) on host cores")
         out["cpu_baseline"] = cpu_baseline(pos, vel, w, args.cpu_steps, args.cpu_budget)
