@@ -199,6 +199,9 @@ def ensemble_bench(e, steps=None):
     ens.run([e["a_init"]] * len(sims), [da] * len(sims), n_steps)
     ens.sync()
     t_ens = time.perf_counter() - t0
+    ens.set_timing(True)                              # per-stage HIP events, 10 more steps
+    ens.run([e["a_end"] - 11 * da] * len(sims), [da] * len(sims), 10)
+    stage = {k: round(ms / max(c, 1), 4) for k, (ms, c) in ens.timing().items()}
     ens.close()
     log(f"ensemble: {t_ens:.3f} s; one at a time")
     t_solo = 0.0
@@ -221,6 +224,7 @@ def ensemble_bench(e, steps=None):
             "value": updates / t_ens, "unit": "particle-updates/s", "seconds": t_ens,
             "ms_per_step": t_ens / n_steps * 1e3,
             "roofline_frac": bytes_step * n_steps / t_ens / 1e9 / HBM_PEAK_GBS,
+            "stages_ms": stage,
             "one_at_a_time": {"value": updates / t_solo, "seconds": t_solo},
             "speedup_vs_one_at_a_time": t_solo / t_ens}
 
