@@ -377,8 +377,12 @@ struct pm_sim {
   unsigned flags = 0;
   double omega_m0 = 0.31, omega_l0 = 0.69, omega_k0 = 0.0, h0 = 0.68;
   double mass = 1.0;
-  void* s = nullptr;
-  void* v = nullptr;
+  // particle state, double-buffered: sb/vb/idb[cur] hold the current state; each gather reads
+  // them and writes the other buffers in the order of the tile bins
+  void* sb[2] = {nullptr, nullptr};
+  void* vb[2] = {nullptr, nullptr};
+  uint32_t* idb[2] = {nullptr, nullptr};
+  int cur = 0;
   void* rho = nullptr;
   void* H = nullptr;  // half spectrum; the potential is written over it
   void* Q = nullptr;
@@ -391,8 +395,8 @@ struct pm_sim {
   // graph replay of consecutive steps: per-step scalars in a device table read at *d_step
   double* d_table = nullptr;   // kTableSteps x {green coefficient, kick, drift}
   int* d_step = nullptr;
-  hipGraphExec_t graph = nullptr;
-  int graph_steps = 0;
+  // graph[c]: kGraphSteps (an even count) steps captured starting from state buffer c
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool timing = false;
   std::vector<PendingEvent> pending;
   double stage_ms[ST_COUNT] = {};
@@ -419,9 +423,12 @@ size_t bins_bytes(int n, int np_side, int prec) {
 
 size_t sim_bytes(int n, int np_side, int prec) {
   const size_t nn = (size_t)n, np = (size_t)np_side * np_side * np_side;
-  return 6 * np * prec + nn * nn * nn * prec /*rho*/ + nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/ +
-         bins_bytes(n, np_side, prec);
+  return 2 * (6 * np * prec + 4 * np) /*two state buffers*/ + nn * nn * nn * prec /*rho*/ +
+         nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/ + bins_bytes(n, np_side, prec);
 }
+
+template <typename R> PState<R> state_buf(const pm_sim* s, int k) { return {(R*)s->sb[k], (R*)s->vb[k], s->idb[k]}; }
+template <typename R> PState<R> cur_state(const pm_sim* s) { return state_buf<R>(s, s->cur); }
 
 // records HIP events around a stage on the owner's stream when its timing is on (the owner is
 // a sim or an ensemble)
@@ -465,9 +472,9 @@ int sim_collect_timing(pm_sim* s) { return collect_timing(s); }
 template <typename R> int sim_bin_impl(pm_sim* s) {
   StageTimer t(s, ST_BIN);
   PM_HIP(hipMemsetAsync(s->B.counts, 0, 2 * sizeof(uint32_t) * s->G.ntiles, s->stream));
-  PM_HIP(bins_count<R>((const R*)s->s, s->L, s->G, s->B, s->stream));
+  PM_HIP(bins_count<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream));
   PM_HIP(bins_scan(s->G, s->B, s->stream));
-  PM_HIP(bins_scatter<R>((const R*)s->s, s->L, s->G, s->B, s->stream));
+  PM_HIP(bins_scatter<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream));
   s->bins_valid = true;
   return PM_OK;
 }
@@ -476,7 +483,7 @@ template <typename R> int sim_deposit_impl(pm_sim* s, int* step = nullptr) {
   if (!s->bins_valid)
     if (int rc = sim_bin_impl<R>(s)) return rc;
   StageTimer t(s, ST_DEPOSIT);
-  PM_HIP(deposit_tiles<R>((const R*)s->s, (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream, step));
+  PM_HIP(deposit_tiles<R>(cur_state<R>(s), (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream, step));
   s->rho_valid = true;
   return PM_OK;
 }
@@ -549,14 +556,16 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
   }
   {
     StageTimer t(s, ST_GKD);
-    PM_HIP(gkd_tiles<R>((R*)s->s, (R*)s->v, (const R*)s->H, sc.kick, sc.drift, s->L, s->G, s->B, tab, s->stream));
+    PM_HIP(gkd_tiles<R>(cur_state<R>(s), state_buf<R>(s, s->cur ^ 1), (const R*)s->H, sc.kick, sc.drift, s->L, s->G,
+                        s->B, tab, s->stream));
   }
+  s->cur ^= 1;
   s->rho_valid = false;
   {
     // the gather counted the entries of the new positions: finish their binning
     StageTimer t(s, ST_BIN);
     PM_HIP(bins_scan(s->G, s->B, s->stream));
-    PM_HIP(bins_scatter<R>((const R*)s->s, s->L, s->G, s->B, s->stream));
+    PM_HIP(bins_scatter<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream));
   }
   if (int rc = sim_deposit_impl<R>(s, table ? s->d_step : nullptr)) return rc;
   return PM_OK;
@@ -565,9 +574,13 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
 constexpr int kTableSteps = 4096;  // steps per uploaded scalar table
 constexpr int kGraphSteps = 8;     // steps captured in one graph
 
-// capture kGraphSteps table-driven steps of this sim into an executable graph (once)
+static_assert(kGraphSteps % 2 == 0 && kTableSteps % 2 == 0, "a graph replay must return to its state buffer");
+
+// capture kGraphSteps table-driven steps of this sim, starting from the current state buffer,
+// into an executable graph (once per buffer)
 template <typename R> int sim_build_graph(pm_sim* s) {
-  if (s->graph) return PM_OK;
+  const int c0 = s->cur;
+  if (s->graph[c0]) return PM_OK;
   if (!s->d_table) {
     PM_HIP(hipMalloc(&s->d_table, sizeof(double) * 3 * kTableSteps));
     PM_HIP(hipMalloc(&s->d_step, sizeof(int)));
@@ -577,18 +590,18 @@ template <typename R> int sim_build_graph(pm_sim* s) {
   int rc = PM_OK;
   for (int k = 0; k < kGraphSteps && rc == PM_OK; ++k) rc = sim_step_impl<R>(s, 1.0, 0.0, true);
   hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  s->cur = c0;  // capture enqueued nothing; an even step count ends on the same buffer anyway
   if (rc != PM_OK) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc;
   }
   if (e != hipSuccess) return fail(PM_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&s->graph, graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&s->graph[c0], graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) {
-    s->graph = nullptr;
+    s->graph[c0] = nullptr;
     return fail(PM_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
   }
-  s->graph_steps = kGraphSteps;
   return PM_OK;
 }
 
@@ -613,7 +626,7 @@ template <typename R> int sim_run_graph(pm_sim* s, double& a, double da, int n_s
     PM_HIP(hipMemcpyAsync(s->d_table, host.data(), sizeof(double) * 3 * chunk, hipMemcpyHostToDevice, s->stream));
     PM_HIP(hipMemcpyAsync(s->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, s->stream));
     int k = 0;
-    for (; k + s->graph_steps <= chunk; k += s->graph_steps) PM_HIP(hipGraphLaunch(s->graph, s->stream));
+    for (; k + kGraphSteps <= chunk; k += kGraphSteps) PM_HIP(hipGraphLaunch(s->graph[s->cur], s->stream));
     for (; k < chunk; ++k)
       if (int rc = sim_step_impl<R>(s, 1.0, 0.0, true)) return rc;
     // the host buffers must outlive the asynchronous copies above
@@ -625,8 +638,10 @@ template <typename R> int sim_run_graph(pm_sim* s, double& a, double da, int n_s
 }
 
 void sim_drop_graph(pm_sim* s) {
-  if (s->graph) (void)hipGraphExecDestroy(s->graph);
-  s->graph = nullptr;
+  for (hipGraphExec_t& g : s->graph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
 }
 
 // use tile shape g from now on (the captured graph holds the old shape's launches)
@@ -664,10 +679,14 @@ template <typename R> int sim_autotune_impl(pm_sim* s, int reps, double* best_ms
     hipError_t e = hipSuccess;
     for (int r = -1; r < reps && e == hipSuccess; ++r) {  // r = -1: warm-up
       if (r == 0) e = hipEventRecord(e0, s->stream);
-      if (!e) e = deposit_tiles<R>((const R*)s->s, (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream);
-      if (!e) e = gkd_tiles<R>((R*)s->s, (R*)s->v, (const R*)s->H, 0.0, 0.0, s->L, s->G, s->B, StepTable{}, s->stream);
+      if (!e) e = deposit_tiles<R>(cur_state<R>(s), (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream);
+      if (!e) {
+        e = gkd_tiles<R>(cur_state<R>(s), state_buf<R>(s, s->cur ^ 1), (const R*)s->H, 0.0, 0.0, s->L, s->G, s->B,
+                         StepTable{}, s->stream);
+        if (!e) s->cur ^= 1;
+      }
       if (!e) e = bins_scan(s->G, s->B, s->stream);
-      if (!e) e = bins_scatter<R>((const R*)s->s, s->L, s->G, s->B, s->stream);
+      if (!e) e = bins_scatter<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream);
     }
     if (!e) e = hipEventRecord(e1, s->stream);
     if (!e) e = hipEventSynchronize(e1);
@@ -763,9 +782,12 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
     return rc;
   };
   if (int rc = get_tables(n_cells, s->prec, &s->tab)) return cleanup(rc);
-  if (hipMalloc(&s->s, 3 * (size_t)s->L.np * pr) != hipSuccess ||
-      hipMalloc(&s->v, 3 * (size_t)s->L.np * pr) != hipSuccess ||
-      hipMalloc(&s->rho, nn * nn * nn * pr) != hipSuccess || hipMalloc(&s->H, nn * nn * nn * pr) != hipSuccess ||
+  bool ok = true;
+  for (int k = 0; k < 2; ++k)
+    ok = ok && hipMalloc(&s->sb[k], 3 * (size_t)s->L.np * pr) == hipSuccess &&
+         hipMalloc(&s->vb[k], 3 * (size_t)s->L.np * pr) == hipSuccess &&
+         hipMalloc(&s->idb[k], (size_t)s->L.np * 4) == hipSuccess;
+  if (!ok || hipMalloc(&s->rho, nn * nn * nn * pr) != hipSuccess || hipMalloc(&s->H, nn * nn * nn * pr) != hipSuccess ||
       hipMalloc(&s->Q, nn * nn * 2 * pr) != hipSuccess)
     return cleanup(fail(PM_E_NOMEM, "device allocation of %zu bytes failed", sim_bytes(n_cells, n_parts_side, s->prec)));
   s->G = make_tiles(n_cells, s->prec);
@@ -780,6 +802,8 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
   // every entry holds a valid slot from the start, counters start at zero
   s->B.done = s->B.block_sums + scan_blocks(gmax);
   if (hipMemset(s->B.entries, 0, 8 * (size_t)s->L.np * 4) != hipSuccess ||
+      hipMemset(s->idb[0], 0, (size_t)s->L.np * 4) != hipSuccess ||
+      hipMemset(s->idb[1], 0, (size_t)s->L.np * 4) != hipSuccess ||
       hipMemset(s->B.done, 0, 4) != hipSuccess ||
       hipMemset(s->B.counts, 0, m * 4) != hipSuccess || hipMemset(s->B.cursors, 0, m * 4) != hipSuccess ||
       hipMemset(s->B.offsets, 0, (m + 1) * 4) != hipSuccess)
@@ -797,8 +821,11 @@ PM_API int pm_sim_destroy(pm_sim* s) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
   }
-  (void)hipFree(s->s);
-  (void)hipFree(s->v);
+  for (int k = 0; k < 2; ++k) {
+    (void)hipFree(s->sb[k]);
+    (void)hipFree(s->vb[k]);
+    (void)hipFree(s->idb[k]);
+  }
   (void)hipFree(s->rho);
   (void)hipFree(s->H);
   (void)hipFree(s->Q);
@@ -807,7 +834,7 @@ PM_API int pm_sim_destroy(pm_sim* s) {
   (void)hipFree(s->B.cursors);
   (void)hipFree(s->B.entries);
   (void)hipFree(s->B.block_sums);
-  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  sim_drop_graph(s);
   (void)hipFree(s->d_table);
   (void)hipFree(s->d_step);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -829,10 +856,7 @@ PM_API int pm_sim_set_mass(pm_sim* s, double mass) {
   if (!(mass > 0.0)) return fail(PM_E_INVALID, "mass must be positive");
   if (mass != s->mass) {
     s->rho_valid = false;
-    if (s->graph) {  // the captured deposit holds the old mass
-      (void)hipGraphExecDestroy(s->graph);
-      s->graph = nullptr;
-    }
+    sim_drop_graph(s);  // the captured deposit holds the old mass
   }
   s->mass = mass;
   return PM_OK;
@@ -840,10 +864,7 @@ PM_API int pm_sim_set_mass(pm_sim* s, double mass) {
 
 PM_API int pm_sim_set_green(pm_sim* s, const float* fgrid) {
   if (!s) return fail(PM_E_INVALID, "null sim");
-  if (fgrid != s->fgrid && s->graph) {  // the captured k-space pass holds the old grid
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-  }
+  if (fgrid != s->fgrid) sim_drop_graph(s);  // the captured k-space pass holds the old grid
   s->fgrid = fgrid;
   return PM_OK;
 }
@@ -897,15 +918,14 @@ PM_API int pm_sim_set_state(pm_sim* s, const void* pos, const void* vel, int dty
   }
   hipError_t e;
   if (s->prec == 4) {
-    e = dtype_bytes == 4 ? sim_set_state<float, float>((const float*)dp, (const float*)dv, (float*)s->s, (float*)s->v,
-                                                       s->L, s->stream)
-                         : sim_set_state<float, double>((const double*)dp, (const double*)dv, (float*)s->s,
-                                                        (float*)s->v, s->L, s->stream);
+    e = dtype_bytes == 4
+            ? sim_set_state<float, float>((const float*)dp, (const float*)dv, cur_state<float>(s), s->L, s->stream)
+            : sim_set_state<float, double>((const double*)dp, (const double*)dv, cur_state<float>(s), s->L, s->stream);
   } else {
-    e = dtype_bytes == 4 ? sim_set_state<double, float>((const float*)dp, (const float*)dv, (double*)s->s,
-                                                        (double*)s->v, s->L, s->stream)
-                         : sim_set_state<double, double>((const double*)dp, (const double*)dv, (double*)s->s,
-                                                         (double*)s->v, s->L, s->stream);
+    e = dtype_bytes == 4
+            ? sim_set_state<double, float>((const float*)dp, (const float*)dv, cur_state<double>(s), s->L, s->stream)
+            : sim_set_state<double, double>((const double*)dp, (const double*)dv, cur_state<double>(s), s->L,
+                                            s->stream);
   }
   hipError_t es = hipStreamSynchronize(s->stream);
   if (tmp) (void)hipFree(tmp);
@@ -932,15 +952,12 @@ PM_API int pm_sim_get_state(pm_sim* s, void* pos, void* vel, int dtype_bytes, in
   }
   hipError_t e;
   if (s->prec == 4) {
-    e = dtype_bytes == 4 ? sim_get_state<float, float>((const float*)s->s, (const float*)s->v, (float*)dp, (float*)dv,
-                                                       s->L, s->stream)
-                         : sim_get_state<float, double>((const float*)s->s, (const float*)s->v, (double*)dp,
-                                                        (double*)dv, s->L, s->stream);
+    e = dtype_bytes == 4 ? sim_get_state<float, float>(cur_state<float>(s), (float*)dp, (float*)dv, s->L, s->stream)
+                         : sim_get_state<float, double>(cur_state<float>(s), (double*)dp, (double*)dv, s->L, s->stream);
   } else {
-    e = dtype_bytes == 4 ? sim_get_state<double, float>((const double*)s->s, (const double*)s->v, (float*)dp,
-                                                        (float*)dv, s->L, s->stream)
-                         : sim_get_state<double, double>((const double*)s->s, (const double*)s->v, (double*)dp,
-                                                         (double*)dv, s->L, s->stream);
+    e = dtype_bytes == 4 ? sim_get_state<double, float>(cur_state<double>(s), (float*)dp, (float*)dv, s->L, s->stream)
+                         : sim_get_state<double, double>(cur_state<double>(s), (double*)dp, (double*)dv, s->L,
+                                                         s->stream);
   }
   if (e == hipSuccess && !on_device) {
     e = hipMemcpyAsync(pos, dp, bytes, hipMemcpyDeviceToHost, s->stream);
@@ -1086,11 +1103,14 @@ PM_API int pm_sim_zeldovich(pm_sim* s, const double* amp, int64_t n_amp, double 
   hipError_t e = hipMemcpyAsync(d_amp, amp, sizeof(double) * need, hipMemcpyHostToDevice, s->stream);
   if (e == hipSuccess) {
     if (s->prec == 4)
-      e = zeldovich_ics<float>(noise, noise_bytes, seed, d_amp, psi_scale, vel_factor, s->L, *t, (float*)s->s,
-                               (float*)s->v, (float*)s->rho, (float*)s->H, (cpx<float>*)s->Q, s->stream);
+      e = zeldovich_ics<float>(noise, noise_bytes, seed, d_amp, psi_scale, vel_factor, s->L, *t,
+                               (float*)s->sb[s->cur], (float*)s->vb[s->cur], (float*)s->rho, (float*)s->H,
+                               (cpx<float>*)s->Q, s->stream);
     else
-      e = zeldovich_ics<double>(noise, noise_bytes, seed, d_amp, psi_scale, vel_factor, s->L, *t, (double*)s->s,
-                                (double*)s->v, (double*)s->rho, (double*)s->H, (cpx<double>*)s->Q, s->stream);
+      e = zeldovich_ics<double>(noise, noise_bytes, seed, d_amp, psi_scale, vel_factor, s->L, *t,
+                                (double*)s->sb[s->cur], (double*)s->vb[s->cur], (double*)s->rho, (double*)s->H,
+                                (cpx<double>*)s->Q, s->stream);
+    if (e == hipSuccess) e = ids_identity(s->idb[s->cur], s->L, s->stream);  // the state is in slot order
   }
   hipError_t es = hipStreamSynchronize(s->stream);
   (void)hipFree(d_amp);
@@ -1186,6 +1206,8 @@ struct pm_ensemble {
 namespace {
 
 constexpr int kEnsTableSteps = 1024;  // steps per uploaded ensemble scalar table
+// the batched kernels find a member's state buffer from the parity of the table's step counter
+static_assert(kEnsTableSteps % 2 == 0, "every table chunk but the last must hold an even step count");
 
 using EnsTimer = StageTimerT<pm_ensemble>;
 
@@ -1231,8 +1253,12 @@ void ens_describe(pm_ensemble* e) {
     m.from_grid = s->fgrid ? 1 : 0;
     m.fgrid = s->fgrid;
     m.local_hist = (s->G.ntx >= 4 && s->G.nty >= 4 && s->G.ntz >= 4) ? 1 : 0;
-    m.s = (float*)s->s;
-    m.v = (float*)s->v;
+    for (int b = 0; b < 2; ++b) {
+      m.s[b] = (float*)s->sb[b];
+      m.v[b] = (float*)s->vb[b];
+      m.id[b] = s->idb[b];
+    }
+    m.par = s->cur;
     m.mass = (float)s->mass;
     m.capacity = s->B.capacity;
     m.L = s->L;
@@ -1315,7 +1341,7 @@ int ens_step(pm_ensemble* e) {
   }
   {
     EnsTimer t(e, ST_DEPOSIT);
-    PM_HIP(batch_deposit(d, e->total[BS_TILES], st));  // also advances the step counter
+    PM_HIP(batch_deposit(d, e->total[BS_TILES], st));
   }
   return PM_OK;
 }
@@ -1487,6 +1513,7 @@ PM_API int pm_ensemble_run(pm_ensemble* e, double* a, const double* da, int n_st
   for (int i = 0; i < nm; ++i) {
     a[i] = acur[i];
     pm_sim* s = e->members[i];
+    s->cur ^= n_steps & 1;  // every step's gather switched the member's state buffer
     s->rho_valid = s->bins_valid = s->phi_valid = true;
   }
   return PM_OK;

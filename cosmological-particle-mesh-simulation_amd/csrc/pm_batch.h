@@ -31,9 +31,12 @@ struct BatchMember {
   int from_grid;
   int local_hist;  // the gather's LDS neighbourhood histogram applies (>= 4 tiles per axis)
   const float* fgrid;
-  // particles: displacements and velocities (3 x N_p each), tile bins
-  float* s;
-  float* v;
+  // particles: the two state buffers (displacements, velocities: 3 x N_p each; ids), the one
+  // holding the current state at step 0 of the call (par), tile bins
+  float* s[2];
+  float* v[2];
+  uint32_t* id[2];
+  int par;
   float mass;
   uint32_t capacity;
   Lattice L;
@@ -53,7 +56,7 @@ struct BatchDesc {
   long off[BS_COUNT][kBatchMax + 1];  // block prefix per stage
   // per-step scalars: table[(step * n_members + member) * 3 + {0: Green, 1: kick, 2: drift}]
   const double* table;
-  int* step;  // advanced by the step's last kernel (the deposit)
+  int* step;  // advanced by the scan, between the gather and the binning
   BatchMember m[kBatchMax];
 };
 

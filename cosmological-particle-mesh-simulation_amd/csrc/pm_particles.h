@@ -34,8 +34,9 @@ struct FastDiv {
 // i along positions[0] (zeldovich.py:79-83), so consecutive particles step along grid axis 0,
 // the slowest one in memory.  On the device the same lattice is stored in cubic blocks of
 // B^3 sites (B = 4: one 64-lane wavefront per block), blocks and sites both ordered with i
-// (grid x) fastest.  The slot <-> (i, j, k) map is arithmetic, so no id array is stored and
-// every particle's lattice point q is known from its slot.
+// (grid x) fastest; the slot <-> (i, j, k) map is arithmetic.  A particle's slot is its id:
+// the state starts in slot order, and each step stores the particles in the order of the tile
+// bins (PState), carrying the ids along.
 struct Lattice {
   int n;        // grid cells per side (N_CELLS)
   int np_side;  // particles per side (N_PARTS)
@@ -82,10 +83,11 @@ struct TileGeom {
   int ntiles;
 };
 
-// Slot entries of every tile: own segment [offsets[2t], offsets[2t+1]) and halo segment
-// [offsets[2t+1], offsets[2t+2]); at most 8 entries per particle.
+// Storage-index entries of every tile: own segment [offsets[t], offsets[t+1]) (all own
+// segments together fill [0, N_p)) and halo segment [offsets[ntiles+t], offsets[ntiles+t+1]);
+// at most 8 entries per particle.
 struct Bins {
-  uint32_t* counts = nullptr;      // 2*ntiles, zero between steps
+  uint32_t* counts = nullptr;      // 2*ntiles (own keys, then halo keys), zero between steps
   uint32_t* offsets = nullptr;     // 2*ntiles + 1
   uint32_t* cursors = nullptr;     // 2*ntiles
   uint32_t* entries = nullptr;     // 8*N_p
@@ -103,26 +105,39 @@ TileGeom make_tiles(int n, int prec_bytes, int tx = 0, int ty = 0, int tz = 0);
 std::vector<TileGeom> tile_candidates(int n, int prec_bytes);
 int scan_blocks(const TileGeom& g);
 
-// ---- resident-state kernels (slot order, displacement storage) -------------------------------
+// Resident particle state: displacements s and velocities v (3 x N_p each, component-major)
+// and the id (lattice slot) of the particle at each storage index.  A sim holds two of them;
+// the gather reads one and writes the other in tile-bin order.
+template <typename R> struct PState {
+  R* s;
+  R* v;
+  uint32_t* id;
+};
+
+// ---- resident-state kernels (displacement storage) -------------------------------------------
+// set: stores slot order (id = storage index); get: writes every particle at its reference index
 template <typename R, typename Tin>
-hipError_t sim_set_state(const Tin* pos, const Tin* vel, R* s, R* v, const Lattice& L, hipStream_t st);
+hipError_t sim_set_state(const Tin* pos, const Tin* vel, const PState<R>& p, const Lattice& L, hipStream_t st);
 template <typename R, typename Tout>
-hipError_t sim_get_state(const R* s, const R* v, Tout* pos, Tout* vel, const Lattice& L, hipStream_t st);
+hipError_t sim_get_state(const PState<R>& p, Tout* pos, Tout* vel, const Lattice& L, hipStream_t st);
+// id = storage index (slot order)
+hipError_t ids_identity(uint32_t* id, const Lattice& L, hipStream_t st);
 
 // ---- tile binning, owner-computes deposit, tile-staged gather + kick + drift ------------------
 template <typename R>
-hipError_t bins_count(const R* s, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
+hipError_t bins_count(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
 hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st);
 template <typename R>
-hipError_t bins_scatter(const R* s, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
+hipError_t bins_scatter(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
 // step (optional): a table-driven step's counter, advanced by the deposit as the step's last kernel
 template <typename R>
-hipError_t deposit_tiles(const R* s, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
+hipError_t deposit_tiles(const PState<R>& p, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
                          hipStream_t st, int* step = nullptr);
-// also counts the entries of the new positions into B.counts
+// Reads `in` through the bins' own entries and writes the updated particles to `out` at their
+// own-entry index (never in place); also counts the entries of the new positions into B.counts.
 template <typename R>
-hipError_t gkd_tiles(R* s, R* v, const R* phi, double kick, double drift, const Lattice& L, const TileGeom& G,
-                     const Bins& B, const StepTable& tab, hipStream_t st);
+hipError_t gkd_tiles(const PState<R>& in, const PState<R>& out, const R* phi, double kick, double drift,
+                     const Lattice& L, const TileGeom& G, const Bins& B, const StepTable& tab, hipStream_t st);
 
 // ---- reference-layout kernels (absolute float32 positions, reference order) ------------------
 hipError_t ref_density(const float* pos, long np, int n, double mass, float* rho, hipStream_t st);

@@ -19,9 +19,11 @@ inline unsigned grid_for(long n, int block = kBlock) {
 // ------------------------------------------------------------------------------------------
 template <typename R, typename Tin>
 __global__ __launch_bounds__(kBlock) void k_set_state(const Tin* __restrict__ pos, const Tin* __restrict__ vel,
-                                                      R* __restrict__ s, R* __restrict__ v, Lattice L) {
+                                                      R* __restrict__ s, R* __restrict__ v, uint32_t* __restrict__ id,
+                                                      Lattice L) {
   const long slot = blockIdx.x * (long)kBlock + threadIdx.x;
   if (slot >= L.np) return;
+  id[slot] = (uint32_t)slot;
   int ijk[3];
   slot_to_ijk(slot, L, ijk[0], ijk[1], ijk[2]);
   const long p = ((long)ijk[0] * L.np_side + ijk[1]) * L.np_side + ijk[2];
@@ -37,22 +39,31 @@ __global__ __launch_bounds__(kBlock) void k_set_state(const Tin* __restrict__ po
   }
 }
 
+// storage index e holds particle id[e]; ids outside the lattice are skipped
 template <typename R, typename Tout>
 __global__ __launch_bounds__(kBlock) void k_get_state(const R* __restrict__ s, const R* __restrict__ v,
-                                                      Tout* __restrict__ pos, Tout* __restrict__ vel, Lattice L) {
-  const long slot = blockIdx.x * (long)kBlock + threadIdx.x;
-  if (slot >= L.np) return;
+                                                      const uint32_t* __restrict__ id, Tout* __restrict__ pos,
+                                                      Tout* __restrict__ vel, Lattice L) {
+  const long e = blockIdx.x * (long)kBlock + threadIdx.x;
+  if (e >= L.np) return;
+  const uint32_t slot = id[e];
+  if (slot >= (uint64_t)L.np) return;
   int ijk[3];
   slot_to_ijk(slot, L, ijk[0], ijk[1], ijk[2]);
   const long p = ((long)ijk[0] * L.np_side + ijk[1]) * L.np_side + ijk[2];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const R x = canonical_pos<R>(lattice_coord<R>(ijk[d], L), s[d * L.np + slot], R(L.n));
+    const R x = canonical_pos<R>(lattice_coord<R>(ijk[d], L), s[d * L.np + e], R(L.n));
     Tout xo = Tout(x);
     if (xo >= Tout(L.n)) xo = Tout(0);
     pos[d * L.np + p] = xo;
-    vel[d * L.np + p] = Tout(v[d * L.np + slot]);
+    vel[d * L.np + p] = Tout(v[d * L.np + e]);
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ids_identity(uint32_t* __restrict__ id, long np) {
+  const long e = blockIdx.x * (long)kBlock + threadIdx.x;
+  if (e < np) id[e] = (uint32_t)e;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -273,29 +284,27 @@ __global__ __launch_bounds__(kBlock) void k_convert(const Tin* __restrict__ in, 
 // host launchers
 // ------------------------------------------------------------------------------------------
 template <typename R, typename Tin>
-hipError_t sim_set_state(const Tin* pos, const Tin* vel, R* s, R* v, const Lattice& L, hipStream_t st) {
-  k_set_state<R, Tin><<<grid_for(L.np), kBlock, 0, st>>>(pos, vel, s, v, L);
+hipError_t sim_set_state(const Tin* pos, const Tin* vel, const PState<R>& p, const Lattice& L, hipStream_t st) {
+  k_set_state<R, Tin><<<grid_for(L.np), kBlock, 0, st>>>(pos, vel, p.s, p.v, p.id, L);
   return hipGetLastError();
 }
 template <typename R, typename Tout>
-hipError_t sim_get_state(const R* s, const R* v, Tout* pos, Tout* vel, const Lattice& L, hipStream_t st) {
-  k_get_state<R, Tout><<<grid_for(L.np), kBlock, 0, st>>>(s, v, pos, vel, L);
+hipError_t sim_get_state(const PState<R>& p, Tout* pos, Tout* vel, const Lattice& L, hipStream_t st) {
+  k_get_state<R, Tout><<<grid_for(L.np), kBlock, 0, st>>>(p.s, p.v, p.id, pos, vel, L);
   return hipGetLastError();
 }
-template hipError_t sim_set_state<float, float>(const float*, const float*, float*, float*, const Lattice&, hipStream_t);
-template hipError_t sim_set_state<float, double>(const double*, const double*, float*, float*, const Lattice&,
-                                                 hipStream_t);
-template hipError_t sim_set_state<double, float>(const float*, const float*, double*, double*, const Lattice&,
-                                                 hipStream_t);
-template hipError_t sim_set_state<double, double>(const double*, const double*, double*, double*, const Lattice&,
-                                                  hipStream_t);
-template hipError_t sim_get_state<float, float>(const float*, const float*, float*, float*, const Lattice&, hipStream_t);
-template hipError_t sim_get_state<float, double>(const float*, const float*, double*, double*, const Lattice&,
-                                                 hipStream_t);
-template hipError_t sim_get_state<double, float>(const double*, const double*, float*, float*, const Lattice&,
-                                                 hipStream_t);
-template hipError_t sim_get_state<double, double>(const double*, const double*, double*, double*, const Lattice&,
-                                                  hipStream_t);
+hipError_t ids_identity(uint32_t* id, const Lattice& L, hipStream_t st) {
+  k_ids_identity<<<grid_for(L.np), kBlock, 0, st>>>(id, L.np);
+  return hipGetLastError();
+}
+#define PM_INST(R, T)                                                                                  \
+  template hipError_t sim_set_state<R, T>(const T*, const T*, const PState<R>&, const Lattice&, hipStream_t); \
+  template hipError_t sim_get_state<R, T>(const PState<R>&, T*, T*, const Lattice&, hipStream_t);
+PM_INST(float, float)
+PM_INST(float, double)
+PM_INST(double, float)
+PM_INST(double, double)
+#undef PM_INST
 
 hipError_t ref_density(const float* pos, long np, int n, double mass, float* rho, hipStream_t st) {
   hipError_t e = hipMemsetAsync(rho, 0, (size_t)n * n * n * sizeof(float), st);

@@ -1,11 +1,15 @@
 // Eulerian tiles for the PM step: per-step counting sort of particle slots into grid tiles,
 // owner-computes CIC deposit and the tile-staged gather + kick + drift (gfx950).
 //
-// Each tile of TX x TY x TZ cells owns two segments of slot entries:
+// Each tile of TX x TY x TZ cells owns two segments of particle entries (storage indices):
 //   own  : particles whose cell floor(x) lies in the tile;
 //   halo : particles of the lower neighbour tiles whose +1 CIC corner falls in this tile
 //          (a particle on the upper face of its tile is entered once more for every tile its
 //          cloud reaches, at most 8 entries in all).
+// All own segments come first, in tile order, then all halo segments: the own entries are a
+// permutation of the particles sorted by tile, and the gather writes each particle's new state
+// at its own-entry index into the other state buffer, so the stored particles follow the tiles
+// from one step to the next (their lattice ids travel with them).
 // The deposit of a tile reads own + halo entries, accumulates the in-tile corners in LDS and
 // writes every cell of the tile with plain stores: no global float atomics, no zero fill.
 // The LDS accumulator is FP64 in both precisions: on gfx950 ds_add_f64 sustains about twice
@@ -48,10 +52,10 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
 // canonical position of a resident particle from its displacement: cell, CIC fractions and
 // lattice point q per axis
 template <typename R>
-__device__ __forceinline__ void cell_and_frac(R sx, R sy, R sz, long slot, const Lattice& L, int c[3], R d[3],
+__device__ __forceinline__ void cell_and_frac(R sx, R sy, R sz, uint32_t id, const Lattice& L, int c[3], R d[3],
                                               R q[3]) {
   int ijk[3];
-  slot_to_ijk(slot, L, ijk[0], ijk[1], ijk[2]);
+  slot_to_ijk(id, L, ijk[0], ijk[1], ijk[2]);
   const R sv[3] = {sx, sy, sz};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -62,10 +66,12 @@ __device__ __forceinline__ void cell_and_frac(R sx, R sy, R sz, long slot, const
   }
 }
 
+// particle at storage index e, lattice id `id`
 template <typename R>
-__device__ __forceinline__ void cell_and_frac(const R* __restrict__ s, long slot, const Lattice& L, int c[3], R d[3]) {
+__device__ __forceinline__ void cell_and_frac(const R* __restrict__ s, long e, uint32_t id, const Lattice& L, int c[3],
+                                              R d[3]) {
   R q[3];
-  cell_and_frac<R>(s[slot], s[L.np + slot], s[2 * L.np + slot], slot, L, c, d, q);
+  cell_and_frac<R>(s[e], s[L.np + e], s[2 * L.np + e], id, L, c, d, q);
 }
 
 // home tile of a cell and whether the cloud reaches the next tile along each axis
@@ -80,14 +86,15 @@ __device__ __forceinline__ void tile_of(const int c[3], const TileGeom& G, int t
   up[2] = (c[2] % TZ == TZ - 1) && G.ntz > 1;
 }
 
-// key of corner-offset o (bit 0: x, 1: y, 2: z) of a particle in tile t: 2*tile (+1 for halo)
+// key of corner-offset o (bit 0: x, 1: y, 2: z) of a particle in tile t: the tile (own
+// segment) for o = 0, ntiles + the tile (halo segment) otherwise
 __device__ __forceinline__ uint32_t target_key(const int t[3], int o, const TileGeom& G) {
   int x = t[0] + (o & 1), y = t[1] + ((o >> 1) & 1), z = t[2] + (o >> 2);
   if (x == G.ntx) x = 0;
   if (y == G.nty) y = 0;
   if (z == G.ntz) z = 0;
   const uint32_t tile = (uint32_t)((z * G.nty + y) * G.ntx + x);
-  return 2u * tile + (o != 0 ? 1u : 0u);
+  return tile + (o != 0 ? (uint32_t)G.ntiles : 0u);
 }
 
 __device__ __forceinline__ bool target_active(const int up[3], int o) {
@@ -123,14 +130,14 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t* ctr, uint32_t key, bool a
 // counting and scattering slot entries
 // ------------------------------------------------------------------------------------------
 template <typename R, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kT) void k_bin_count(const R* __restrict__ s, Lattice L, TileGeom G,
-                                                  uint32_t* __restrict__ counts) {
-  const long slot = blockIdx.x * (long)kT + threadIdx.x;
-  const bool valid = slot < L.np;
+__global__ __launch_bounds__(kT) void k_bin_count(const R* __restrict__ s, const uint32_t* __restrict__ id, Lattice L,
+                                                  TileGeom G, uint32_t* __restrict__ counts) {
+  const long e = blockIdx.x * (long)kT + threadIdx.x;
+  const bool valid = e < L.np;
   int c[3] = {0, 0, 0}, t[3] = {0, 0, 0}, up[3] = {0, 0, 0};
   R d[3];
   if (valid) {
-    cell_and_frac<R>(s, slot, L, c, d);
+    cell_and_frac<R>(s, e, id[e], L, c, d);
     tile_of<TX, TY, TZ>(c, G, t, up);
   }
 #pragma unroll 1
@@ -141,15 +148,16 @@ __global__ __launch_bounds__(kT) void k_bin_count(const R* __restrict__ s, Latti
 }
 
 template <typename R, int TX, int TY, int TZ>
-__device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const Lattice& L, const TileGeom& G,
+__device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const uint32_t* __restrict__ id,
+                                                 const Lattice& L, const TileGeom& G,
                                                  const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursors,
                                                  uint32_t* __restrict__ entries, uint32_t capacity, long blk) {
-  const long slot = blk * (long)kT + threadIdx.x;
+  const long slot = blk * (long)kT + threadIdx.x;  // storage index
   const bool valid = slot < L.np;
   int c[3] = {0, 0, 0}, t[3] = {0, 0, 0}, up[3] = {0, 0, 0};
   R d[3];
   if (valid) {
-    cell_and_frac<R>(s, slot, L, c, d);
+    cell_and_frac<R>(s, slot, id[slot], L, c, d);
     tile_of<TX, TY, TZ>(c, G, t, up);
   }
   // home tile (o = 0) of every particle: its segment bounds load alongside the cursor atomic
@@ -182,10 +190,11 @@ __device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const 
 }
 
 template <typename R, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, Lattice L, TileGeom G,
-                                                    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursors,
-                                                    uint32_t* __restrict__ entries, uint32_t capacity) {
-  bin_scatter_body<R, TX, TY, TZ>(s, L, G, offsets, cursors, entries, capacity, blockIdx.x);
+__global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, const uint32_t* __restrict__ id,
+                                                    Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
+                                                    uint32_t* __restrict__ cursors, uint32_t* __restrict__ entries,
+                                                    uint32_t capacity) {
+  bin_scatter_body<R, TX, TY, TZ>(s, id, L, G, offsets, cursors, entries, capacity, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -308,8 +317,9 @@ constexpr uint32_t kNone = 0xffffffffu;
 template <int TX, int TY, int TZ> constexpr int deposit_lds_cells() { return (TX + 3) * TY * TZ; }
 
 template <typename R, int TX, int TY, int TZ>
-__device__ __forceinline__ void deposit_body(const R* __restrict__ s, R* __restrict__ rho, R mass, const Lattice& L,
-                                             const TileGeom& G, const uint32_t* __restrict__ offsets,
+__device__ __forceinline__ void deposit_body(const R* __restrict__ s, const uint32_t* __restrict__ id,
+                                             R* __restrict__ rho, R mass, const Lattice& L, const TileGeom& G,
+                                             const uint32_t* __restrict__ offsets,
                                              const uint32_t* __restrict__ entries, uint32_t capacity, int tile,
                                              double* acc) {
   constexpr int CELLS = TX * TY * TZ;
@@ -321,32 +331,38 @@ __device__ __forceinline__ void deposit_body(const R* __restrict__ s, R* __restr
   const int T[3] = {TX, TY, TZ};
   const int n = L.n;
   const long np = L.np;
-  const uint32_t end = min(offsets[2 * tile + 2], capacity);
-  const uint32_t beg = min(offsets[2 * tile], end);
-  uint32_t sl[E];
+  // own entries [ob, oe) and halo entries [hb, he), read as one list of n_own + n_halo
+  const uint32_t nt = (uint32_t)G.ntiles;
+  const uint32_t oe = min(offsets[tile + 1], capacity), ob = min(offsets[tile], oe);
+  const uint32_t he = min(offsets[nt + tile + 1], capacity), hb = min(offsets[nt + tile], he);
+  const uint32_t n_own = oe - ob, total = n_own + (he - hb);
+  uint32_t sl[E], pid[E];
   R sv[E][3];
   auto load = [&](uint32_t base) {
 #pragma unroll
     for (int u = 0; u < E; ++u) {
-      const uint32_t e = base + u * kT + threadIdx.x;
-      sl[u] = e < end ? min(entries[e], (uint32_t)(np - 1)) : kNone;
+      const uint32_t i = base + u * kT + threadIdx.x;
+      const uint32_t e = i < n_own ? ob + i : hb + (i - n_own);
+      sl[u] = i < total ? min(entries[e], (uint32_t)(np - 1)) : kNone;
     }
 #pragma unroll
-    for (int u = 0; u < E; ++u)
+    for (int u = 0; u < E; ++u) {
+      pid[u] = sl[u] != kNone ? id[sl[u]] : 0u;
 #pragma unroll
       for (int a = 0; a < 3; ++a) sv[u][a] = sl[u] != kNone ? s[a * np + sl[u]] : R(0);
+    }
   };
-  load(beg);  // the first batch is in flight while the accumulator is cleared
+  load(0);  // the first batch is in flight while the accumulator is cleared
   for (int i = threadIdx.x; i < PADDED; i += kT) acc[i] = 0.0;
   __syncthreads();
-  for (uint32_t base = beg; base < end; base += kT * E) {
-    if (base != beg) load(base);
+  for (uint32_t base = 0; base < total; base += kT * E) {
+    if (base != 0) load(base);
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       if (sl[u] == kNone) continue;
       int c[3];
       R d[3], q[3];
-      cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], sl[u], L, c, d, q);
+      cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], pid[u], L, c, d, q);
       int r0[3], r1[3];  // tile-relative coordinates of the two corners per axis (-1: outside)
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -394,12 +410,13 @@ __device__ __forceinline__ void deposit_body(const R* __restrict__ s, R* __restr
 // step: when set, the last kernel of a table-driven step advances the table's step counter
 // (no kernel of the step reads it after the gather)
 template <typename R, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, R* __restrict__ rho, R mass,
-                                                      Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
+__global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, const uint32_t* __restrict__ id,
+                                                      R* __restrict__ rho, R mass, Lattice L, TileGeom G,
+                                                      const uint32_t* __restrict__ offsets,
                                                       const uint32_t* __restrict__ entries, uint32_t capacity,
                                                       int* step) {
   __shared__ double acc[deposit_lds_cells<TX, TY, TZ>()];
-  deposit_body<R, TX, TY, TZ>(s, rho, mass, L, G, offsets, entries, capacity, xcd_tile(blockIdx.x, gridDim.x),
+  deposit_body<R, TX, TY, TZ>(s, id, rho, mass, L, G, offsets, entries, capacity, xcd_tile(blockIdx.x, gridDim.x),
                               acc);
   if (step && blockIdx.x == 0 && threadIdx.x == 0) *step += 1;
 }
@@ -422,8 +439,8 @@ constexpr int kGkdE = 2;  // particles per thread and batch in the gather
 // own-entry range [beg, end) of a tile, and a thread's first batch of its slot entries
 __device__ __forceinline__ void tile_range(const uint32_t* __restrict__ offsets, uint32_t capacity, int tile,
                                            uint32_t& beg, uint32_t& end) {
-  end = min(offsets[2 * tile + 1], capacity);
-  beg = min(offsets[2 * tile], end);
+  end = min(offsets[tile + 1], capacity);
+  beg = min(offsets[tile], end);
 }
 __device__ __forceinline__ void first_entries(const uint32_t* __restrict__ entries, uint32_t beg, uint32_t end,
                                               long np, uint32_t (&sl)[kGkdE]) {
@@ -434,12 +451,23 @@ __device__ __forceinline__ void first_entries(const uint32_t* __restrict__ entri
   }
 }
 
+// Particle state the gather reads (at storage index) and writes (at own-entry index)
+template <typename R> struct GkdState {
+  const R* s_in;
+  const R* v_in;
+  const uint32_t* id_in;
+  R* s_out;
+  R* v_out;
+  uint32_t* id_out;
+};
+
 // Gather + kick + drift of one tile's own particles.  [beg, end) is the tile's own-entry range
 // and sl0 the thread's first batch of entries (fetched by the caller before the tile's
-// potential box, so the two dependent load chains overlap).
+// potential box, so the two dependent load chains overlap).  Own entry e's particle is
+// written to index e of the output state.
 template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
-__device__ __forceinline__ void gkd_tile(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi, R kick,
-                                         R drift, const Lattice& L, const TileGeom& G,
+__device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restrict__ phi, R kick, R drift,
+                                         const Lattice& L, const TileGeom& G,
                                          const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
                                          int tile, uint32_t beg, uint32_t end, const uint32_t (&sl0)[kGkdE], R* box,
                                          uint32_t* hist) {
@@ -462,7 +490,7 @@ __device__ __forceinline__ void gkd_tile(R* __restrict__ s, R* __restrict__ v, c
   if (LOCAL_HIST && threadIdx.x < 128) hist[threadIdx.x] = 0u;
   // The first batch of entries is fetched before the potential box, the batch's particle
   // state together with the box, so the two dependent load chains overlap.
-  uint32_t sl[E];
+  uint32_t sl[E], pid[E];
 #pragma unroll
   for (int u = 0; u < E; ++u) sl[u] = sl0[u];
   R sv[E][3], vv[E][3];
@@ -475,12 +503,14 @@ __device__ __forceinline__ void gkd_tile(R* __restrict__ s, R* __restrict__ v, c
   };
   auto load_state = [&]() {
 #pragma unroll
-    for (int u = 0; u < E; ++u)
+    for (int u = 0; u < E; ++u) {
+      pid[u] = sl[u] != kNone ? ps.id_in[sl[u]] : 0u;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        sv[u][a] = sl[u] != kNone ? s[a * np + sl[u]] : R(0);
-        vv[u][a] = sl[u] != kNone ? v[a * np + sl[u]] : R(0);
+        sv[u][a] = sl[u] != kNone ? ps.s_in[a * np + sl[u]] : R(0);
+        vv[u][a] = sl[u] != kNone ? ps.v_in[a * np + sl[u]] : R(0);
       }
+    }
   };
   // potential of the tile and its -1/+2 stencil margin (periodic).  Each wavefront loads whole
   // z planes of the box (BY x BX values in PL loads of 64 lanes): a lane's in-plane offsets are
@@ -526,12 +556,13 @@ __device__ __forceinline__ void gkd_tile(R* __restrict__ s, R* __restrict__ v, c
     }
 #pragma unroll
     for (int u = 0; u < E; ++u) {
-      const bool valid = sl[u] != kNone;
+      const uint32_t eo = base + u * kT + threadIdx.x;  // own-entry index = output index
+      const bool valid = sl[u] != kNone && eo < (uint32_t)np;
       int tn[3] = {0, 0, 0}, up[3] = {0, 0, 0};
       if (valid) {
         int c[3];
         R d[3], q[3];
-        cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], sl[u], L, c, d, q);
+        cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], pid[u], L, c, d, q);
         // tile-relative cell (box coordinate of the cell is that + 1).  Own entries were binned
         // from these same positions, so the cell lies in the tile; the clamp only keeps a
         // corrupted state inside the box.
@@ -574,10 +605,11 @@ __device__ __forceinline__ void gkd_tile(R* __restrict__ s, R* __restrict__ v, c
         for (int a = 0; a < 3; ++a) {
           const R va = vv[u][a] + kick * (R(0.5) * g[a]);
           const R sa = wrap_disp<R>(sv[u][a] + drift * va, nr);
-          v[a * np + sl[u]] = va;
-          s[a * np + sl[u]] = sa;
+          ps.v_out[a * np + eo] = va;
+          ps.s_out[a * np + eo] = sa;
           cn[a] = cell_of<R>(canonical_pos<R>(q[a], sa, nr), n);
         }
+        ps.id_out[eo] = pid[u];
         tile_of<TX, TY, TZ>(cn, G, tn, up);
         if (LOCAL_HIST) {
           // new home tile relative to this one, folded into [-1, 2] per axis; when it lies in
@@ -627,28 +659,26 @@ __device__ __forceinline__ void gkd_tile(R* __restrict__ s, R* __restrict__ v, c
         const int cell = threadIdx.x >> 1, kind = threadIdx.x & 1;
         const int rx = cell & 3, ry = (cell >> 2) & 3, rz = cell >> 4;
         const int x = fold(tix + rx - 1, G.ntx), y = fold(tiy + ry - 1, G.nty), z = fold(tiz + rz - 1, G.ntz);
-        atomicAdd(&counts[2u * (uint32_t)((z * G.nty + y) * G.ntx + x) + kind], cnt);
+        atomicAdd(&counts[(uint32_t)((z * G.nty + y) * G.ntx + x) + (kind ? (uint32_t)G.ntiles : 0u)], cnt);
       }
     }
   }
 }
 
 template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
-__device__ __forceinline__ void gkd_body(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi, R kick,
-                                         R drift, const Lattice& L, const TileGeom& G,
-                                         const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ entries,
-                                         uint32_t* __restrict__ counts, uint32_t capacity, int tile, R* box,
-                                         uint32_t* hist) {
+__device__ __forceinline__ void gkd_body(const GkdState<R>& P, const R* __restrict__ phi, R kick, R drift,
+                                         const Lattice& L, const TileGeom& G, const uint32_t* __restrict__ offsets,
+                                         const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
+                                         uint32_t capacity, int tile, R* box, uint32_t* hist) {
   uint32_t beg, end, sl[kGkdE];
   tile_range(offsets, capacity, tile, beg, end);
   first_entries(entries, beg, end, L.np, sl);
-  gkd_tile<R, TX, TY, TZ, LOCAL_HIST>(s, v, phi, kick, drift, L, G, entries, counts, tile, beg, end, sl, box, hist);
+  gkd_tile<R, TX, TY, TZ, LOCAL_HIST>(P, phi, kick, drift, L, G, entries, counts, tile, beg, end, sl, box, hist);
 }
 
 template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
-__global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restrict__ v, const R* __restrict__ phi,
-                                                  R kick, R drift, Lattice L, TileGeom G,
-                                                  const uint32_t* __restrict__ offsets,
+__global__ __launch_bounds__(kT) void k_gkd_tiles(GkdState<R> P, const R* __restrict__ phi, R kick, R drift,
+                                                  Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
                                                   uint32_t capacity, StepTable tab) {
   __shared__ R box[gkd_box_cells<TX, TY, TZ>()];
@@ -658,7 +688,7 @@ __global__ __launch_bounds__(kT) void k_gkd_tiles(R* __restrict__ s, R* __restri
     kick = R(tab.table[3 * k + 1]);
     drift = R(tab.table[3 * k + 2]);
   }
-  gkd_body<R, TX, TY, TZ, LOCAL_HIST>(s, v, phi, kick, drift, L, G, offsets, entries, counts, capacity,
+  gkd_body<R, TX, TY, TZ, LOCAL_HIST>(P, phi, kick, drift, L, G, offsets, entries, counts, capacity,
                                       xcd_tile(blockIdx.x, gridDim.x), box, hist);
 }
 
@@ -676,14 +706,23 @@ __global__ __launch_bounds__(kT) void k_batch_gkd(const BatchDesc* __restrict__ 
   const int k = batch_member(D, BS_TILES, b);
   const BatchMember& m = D->m[k];
   const int tile = (int)(b - D->off[BS_TILES][k]);
-  const double* sc = D->table + 3 * ((long)(*D->step) * D->n_members + k);
+  const int step = *D->step;
+  const double* sc = D->table + 3 * ((long)step * D->n_members + k);
   const float kick = float(sc[1]), drift = float(sc[2]);
+  const int in = (m.par + step) & 1;
+  const GkdState<float> ps{m.s[in], m.v[in], m.id[in], m.s[in ^ 1], m.v[in ^ 1], m.id[in ^ 1]};
   if (m.local_hist)
-    gkd_body<float, kBT, kBT, kBT, true>(m.s, m.v, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
+    gkd_body<float, kBT, kBT, kBT, true>(ps, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
                                          m.offsets, m.entries, m.counts, m.capacity, tile, box, hist);
   else
-    gkd_body<float, kBT, kBT, kBT, false>(m.s, m.v, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
+    gkd_body<float, kBT, kBT, kBT, false>(ps, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
                                           m.offsets, m.entries, m.counts, m.capacity, tile, box, hist);
+}
+
+// state buffer the binning and the deposit read: the gather's output (the scan has advanced
+// the step counter since the gather read it)
+__device__ __forceinline__ int batch_state(const BatchDesc* D, const BatchMember& m) {
+  return (m.par + *D->step) & 1;
 }
 
 __global__ __launch_bounds__(kT) void k_batch_deposit(const BatchDesc* __restrict__ D) {
@@ -691,16 +730,17 @@ __global__ __launch_bounds__(kT) void k_batch_deposit(const BatchDesc* __restric
   const long b = xcd_tile(blockIdx.x, gridDim.x);
   const int k = batch_member(D, BS_TILES, b);
   const BatchMember& m = D->m[k];
-  deposit_body<float, kBT, kBT, kBT>(m.s, m.rho, m.mass, m.L, m.G, m.offsets, m.entries, m.capacity,
+  const int cur = batch_state(D, m);
+  deposit_body<float, kBT, kBT, kBT>(m.s[cur], m.id[cur], m.rho, m.mass, m.L, m.G, m.offsets, m.entries, m.capacity,
                                      (int)(b - D->off[BS_TILES][k]), acc);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *D->step += 1;  // the step's last kernel
 }
 
 __global__ __launch_bounds__(kT) void k_batch_scatter(const BatchDesc* __restrict__ D) {
   const long b = blockIdx.x;
   const int k = batch_member(D, BS_SCATTER, b);
   const BatchMember& m = D->m[k];
-  bin_scatter_body<float, kBT, kBT, kBT>(m.s, m.L, m.G, m.offsets, m.cursors, m.entries, m.capacity,
+  const int cur = batch_state(D, m);
+  bin_scatter_body<float, kBT, kBT, kBT>(m.s[cur], m.id[cur], m.L, m.G, m.offsets, m.cursors, m.entries, m.capacity,
                                          b - D->off[BS_SCATTER][k]);
 }
 
@@ -720,6 +760,9 @@ __global__ __launch_bounds__(kT) void k_batch_scan_apply(const BatchDesc* __rest
   const int k = batch_member(D, BS_SCAN, b);
   const BatchMember& m = D->m[k];
   scan_apply_body(m.counts, 2L * m.G.ntiles, m.block_sums, m.offsets, m.cursors, b - D->off[BS_SCAN][k], sh);
+  // the last kernel that reads the step's scalars was the gather: advance the counter here, so
+  // the binning and the deposit find the gather's output buffer from it
+  if (b == 0 && threadIdx.x == 0) *D->step += 1;
 }
 
 }  // namespace
@@ -784,8 +827,8 @@ int scan_blocks(const TileGeom& g) { return (int)((2L * g.ntiles + kScanChunk - 
   } while (0)
 
 template <typename R>
-hipError_t bins_count(const R* s, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st) {
-#define PM_L(X, Y, Z) k_bin_count<R, X, Y, Z><<<blocks_for(L.np), kT, 0, st>>>(s, L, G, B.counts)
+hipError_t bins_count(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st) {
+#define PM_L(X, Y, Z) k_bin_count<R, X, Y, Z><<<blocks_for(L.np), kT, 0, st>>>(p.s, p.id, L, G, B.counts)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
   return hipGetLastError();
@@ -800,36 +843,40 @@ hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st) {
 }
 
 template <typename R>
-hipError_t bins_scatter(const R* s, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st) {
-#define PM_L(X, Y, Z) \
-  k_bin_scatter<R, X, Y, Z><<<blocks_for(L.np), kT, 0, st>>>(s, L, G, B.offsets, B.cursors, B.entries, B.capacity)
+hipError_t bins_scatter(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st) {
+#define PM_L(X, Y, Z)                                                                                          \
+  k_bin_scatter<R, X, Y, Z><<<blocks_for(L.np), kT, 0, st>>>(p.s, p.id, L, G, B.offsets, B.cursors, B.entries, \
+                                                             B.capacity)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t deposit_tiles(const R* s, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
+hipError_t deposit_tiles(const PState<R>& p, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
                          hipStream_t st, int* step) {
-#define PM_L(X, Y, Z) \
-  k_deposit_tiles<R, X, Y, Z><<<G.ntiles, kT, 0, st>>>(s, rho, R(mass), L, G, B.offsets, B.entries, B.capacity, step)
+#define PM_L(X, Y, Z)                                                                                          \
+  k_deposit_tiles<R, X, Y, Z><<<G.ntiles, kT, 0, st>>>(p.s, p.id, rho, R(mass), L, G, B.offsets, B.entries,    \
+                                                       B.capacity, step)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t gkd_tiles(R* s, R* v, const R* phi, double kick, double drift, const Lattice& L, const TileGeom& G,
-                     const Bins& B, const StepTable& tab, hipStream_t st) {
+hipError_t gkd_tiles(const PState<R>& in, const PState<R>& out, const R* phi, double kick, double drift,
+                     const Lattice& L, const TileGeom& G, const Bins& B, const StepTable& tab, hipStream_t st) {
+  if (in.s == out.s || in.v == out.v || in.id == out.id) return hipErrorInvalidValue;  // never in place
   const bool local_hist = G.ntx >= 4 && G.nty >= 4 && G.ntz >= 4;
-#define PM_L(X, Y, Z)                                                                                            \
-  do {                                                                                                           \
-    if (local_hist)                                                                                              \
-      k_gkd_tiles<R, X, Y, Z, true><<<G.ntiles, kT, 0, st>>>(s, v, phi, R(kick), R(drift), L, G, B.offsets,      \
-                                                             B.entries, B.counts, B.capacity, tab);              \
-    else                                                                                                         \
-      k_gkd_tiles<R, X, Y, Z, false><<<G.ntiles, kT, 0, st>>>(s, v, phi, R(kick), R(drift), L, G, B.offsets,     \
-                                                              B.entries, B.counts, B.capacity, tab);             \
+  const GkdState<R> ps{in.s, in.v, in.id, out.s, out.v, out.id};
+#define PM_L(X, Y, Z)                                                                                             \
+  do {                                                                                                            \
+    if (local_hist)                                                                                               \
+      k_gkd_tiles<R, X, Y, Z, true><<<G.ntiles, kT, 0, st>>>(ps, phi, R(kick), R(drift), L, G, B.offsets,         \
+                                                             B.entries, B.counts, B.capacity, tab);               \
+    else                                                                                                          \
+      k_gkd_tiles<R, X, Y, Z, false><<<G.ntiles, kT, 0, st>>>(ps, phi, R(kick), R(drift), L, G, B.offsets,        \
+                                                              B.entries, B.counts, B.capacity, tab);              \
   } while (0)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
@@ -875,12 +922,14 @@ hipError_t batch_scan(const BatchDesc* d, long total, int n_members, hipStream_t
 }
 
 #define PM_INST(R)                                                                                           \
-  template hipError_t bins_count<R>(const R*, const Lattice&, const TileGeom&, const Bins&, hipStream_t);    \
-  template hipError_t bins_scatter<R>(const R*, const Lattice&, const TileGeom&, const Bins&, hipStream_t);  \
-  template hipError_t deposit_tiles<R>(const R*, R*, double, const Lattice&, const TileGeom&, const Bins&,   \
-                                       hipStream_t, int*);                                                   \
-  template hipError_t gkd_tiles<R>(R*, R*, const R*, double, double, const Lattice&, const TileGeom&,        \
-                                   const Bins&, const StepTable&, hipStream_t);
+  template hipError_t bins_count<R>(const PState<R>&, const Lattice&, const TileGeom&, const Bins&,         \
+                                    hipStream_t);                                                            \
+  template hipError_t bins_scatter<R>(const PState<R>&, const Lattice&, const TileGeom&, const Bins&,       \
+                                      hipStream_t);                                                          \
+  template hipError_t deposit_tiles<R>(const PState<R>&, R*, double, const Lattice&, const TileGeom&,       \
+                                       const Bins&, hipStream_t, int*);                                      \
+  template hipError_t gkd_tiles<R>(const PState<R>&, const PState<R>&, const R*, double, double,            \
+                                   const Lattice&, const TileGeom&, const Bins&, const StepTable&, hipStream_t);
 PM_INST(float)
 PM_INST(double)
 #undef PM_INST

@@ -40,11 +40,12 @@ def test_pure_host_entry_points():
     assert abs(lib.pm_f(0.5, 0.3, 0.7, 0.0) - 1 / (0.5 * (0.3 / 0.125 + 0.7) ** 0.5)) < 1e-14
     b = ctypes.c_size_t()
     assert lib.pm_sim_footprint(512, 256, 32, ctypes.byref(b)) == 0
-    # particles + rho + half spectrum + Nyquist plane + tile bins (8 entries per particle,
+    # two particle state buffers (displacements, velocities, ids) + rho + half spectrum +
+    # Nyquist plane + tile bins (8 entries per particle,
     # counts/offsets/cursors and scan spine sized for the tile shape with the most tiles,
     # 8^3: 2 * 64^3 counters in chunks of 4096, plus the spine's ticket counter)
     ntiles = (512 // 8) ** 3
     bins = 4 * (3 * 2 * ntiles + 1 + 2 * ntiles // 4096 + 1) + 4 * 8 * 256 ** 3
-    assert b.value == 6 * 256 ** 3 * 4 + 2 * 512 ** 3 * 4 + 512 * 512 * 2 * 4 + bins
+    assert b.value == 2 * (6 * 4 + 4) * 256 ** 3 + 2 * 512 ** 3 * 4 + 512 * 512 * 2 * 4 + bins
     assert lib.pm_sim_footprint(512, 256, 16, ctypes.byref(b)) != 0
     assert lib.pm_supported_grid(512) == 1 and lib.pm_supported_grid(100) == 0
