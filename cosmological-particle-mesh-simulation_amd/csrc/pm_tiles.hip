@@ -28,6 +28,7 @@ namespace pm {
 namespace {
 
 constexpr int kT = 256;  // threads per block
+constexpr uint32_t kNone = 0xffffffffu;  // no entry / empty slot
 
 inline unsigned blocks_for(long n, int b = kT) {
   const long g = (n + b - 1) / b;
@@ -147,6 +148,65 @@ __global__ __launch_bounds__(kT) void k_bin_count(const R* __restrict__ s, const
   }
 }
 
+// Places reserved for the active lanes' keys: lanes sharing a key get consecutive places
+// from one cursor atomic issued by the lowest such lane.  The first U distinct keys of the
+// wavefront take straight-line code whose atomics stay in flight until finish() reads them;
+// further keys (rare) go through a loop that waits for each.
+template <int U> struct WaveReserve {
+  uint32_t res[U];  // atomic results, held by the leader lanes
+  int grp;          // group of this lane (U: loop path, place in `direct`)
+  int lead;         // leader lane of the group
+  uint32_t pre;     // rank among the lanes of the group
+  uint32_t direct;
+
+  __device__ __forceinline__ void issue(uint32_t* __restrict__ cursors, uint32_t key, bool active) {
+    const int lane = __lane_id();
+    unsigned long long pending = __ballot(active);
+    grp = U;
+    lead = lane;
+    pre = 0u;
+    direct = 0u;
+#pragma unroll
+    for (int g = 0; g < U; ++g) {
+      res[g] = 0u;
+      if (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lkey = __shfl(key, leader);
+        const unsigned long long same = __ballot(active && key == lkey) & pending;
+        if (lane == leader) res[g] = atomicAdd(&cursors[lkey], (uint32_t)__popcll(same));
+        if ((same >> lane) & 1ull) {
+          grp = g;
+          lead = leader;
+          pre = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        }
+        pending &= ~same;
+      }
+    }
+    while (pending) {
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const uint32_t lkey = __shfl(key, leader);
+      const unsigned long long same = __ballot(active && key == lkey) & pending;
+      uint32_t base = 0u;
+      if (lane == leader) base = atomicAdd(&cursors[lkey], (uint32_t)__popcll(same));
+      base = __shfl(base, leader);
+      if ((same >> lane) & 1ull) direct = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+      pending &= ~same;
+    }
+  }
+
+  __device__ __forceinline__ uint32_t finish() const {
+    uint32_t b = direct;
+#pragma unroll
+    for (int g = 0; g < U; ++g) {
+      const uint32_t v = (uint32_t)__shfl((int)res[g], lead);
+      if (grp == g) b = v + pre;
+    }
+    return b;
+  }
+};
+
+// Scatter of every particle into its tile entries (home tile, and the halo lists of the
+// tiles its cloud reaches), with the cursor atomics of all its keys in flight together.
 template <typename R, int TX, int TY, int TZ>
 __device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const uint32_t* __restrict__ id,
                                                  const Lattice& L, const TileGeom& G,
@@ -160,31 +220,36 @@ __device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const 
     cell_and_frac<R>(s, slot, id[slot], L, c, d);
     tile_of<TX, TY, TZ>(c, G, t, up);
   }
-  // home tile (o = 0) of every particle: its segment bounds load alongside the cursor atomic
-  {
-    const uint32_t key = valid ? target_key(t, 0, G) : 0u;
-    uint32_t beg = 0, lim = 0;
-    if (valid) {
-      beg = offsets[key];
-      lim = offsets[key + 1];
-    }
-    const uint32_t rank = wave_add<true>(cursors, key, valid);
-    if (valid) {
-      const uint32_t pos = beg + rank;
-      if (pos < lim && pos < capacity) entries[pos] = (uint32_t)slot;
+  // halo keys only in wavefronts that have any (wave-uniform)
+  const bool any_halo = __ballot(valid && (up[0] | up[1] | up[2])) != 0ull;
+  uint32_t key[8], beg[8], lim[8];
+  bool act[8];
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    act[o] = (o == 0 || any_halo) && valid && target_active(up, o);
+    key[o] = act[o] ? target_key(t, o, G) : 0u;
+    beg[o] = lim[o] = 0u;
+    if (act[o]) {  // segment bounds load alongside the atomics
+      beg[o] = offsets[key[o]];
+      lim[o] = offsets[key[o] + 1];
     }
   }
-  // halo entries (clouds reaching the next tile) are rare: only waves that have any
-  if (__ballot(valid && (up[0] | up[1] | up[2]))) {
-#pragma unroll 1
+  WaveReserve<2> home;
+  home.issue(cursors, key[0], act[0]);
+  WaveReserve<1> halo[7];
+  if (any_halo) {
+#pragma unroll
+    for (int o = 1; o < 8; ++o) halo[o - 1].issue(cursors, key[o], act[o]);
+  }
+  if (act[0]) {
+    const uint32_t pos = beg[0] + home.finish();
+    if (pos < lim[0] && pos < capacity) entries[pos] = (uint32_t)slot;
+  }
+  if (any_halo) {
+#pragma unroll
     for (int o = 1; o < 8; ++o) {
-      const bool act = valid && target_active(up, o);
-      const uint32_t key = act ? target_key(t, o, G) : 0u;
-      const uint32_t rank = wave_add<true>(cursors, key, act);
-      if (act) {
-        const uint32_t pos = offsets[key] + rank;
-        if (pos < offsets[key + 1] && pos < capacity) entries[pos] = (uint32_t)slot;
-      }
+      const uint32_t pos = beg[o] + halo[o - 1].finish();
+      if (act[o] && pos < lim[o] && pos < capacity) entries[pos] = (uint32_t)slot;
     }
   }
 }
@@ -310,8 +375,6 @@ __global__ __launch_bounds__(kT) void k_scan_apply(uint32_t* __restrict__ counts
 // The entry and displacement loads of E particles per thread are issued together before any
 // is used, so each pass over a tile's entries costs two memory latencies, not 2E.
 // ------------------------------------------------------------------------------------------
-constexpr uint32_t kNone = 0xffffffffu;
-
 // row pitch TX + 3: consecutive lattice particles (2 cells apart) then spread over the LDS
 // banks instead of sharing the few that a power-of-two pitch leaves them
 template <int TX, int TY, int TZ> constexpr int deposit_lds_cells() { return (TX + 3) * TY * TZ; }
