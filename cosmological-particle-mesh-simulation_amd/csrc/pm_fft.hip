@@ -310,8 +310,9 @@ __device__ __forceinline__ PencilAddr<R> pencil_addr(cpx<R>* H, cpx<R>* Q, const
 using u32x2 = unsigned int __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* base, long bytes) {
   // 0x00020000: the buffer-resource format word of the gfx9 family (raw 32-bit dwords)
+  // num_records is an unsigned 32-bit byte count; the register pass keeps every offset below 4 GiB
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
-                                           (int)(bytes < 0x7fffffffL ? bytes : 0x7fffffffL), 0x00020000);
+                                           (int)(uint32_t)(bytes < 0xffffffffL ? bytes : 0xffffffffL), 0x00020000);
 }
 __device__ __forceinline__ cpx<float> buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
   return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
@@ -389,9 +390,10 @@ __device__ __forceinline__ void pencil_load(const PencilAddr<R>& p, cpx<R>* lds,
 template <typename R, int N> constexpr bool reg_transform() {
   constexpr int W = pencil_w<R, N>();
   constexpr int TT = kThreads / W;
-  // (the raw-buffer addressing of the register pass covers arrays below 2 GiB: N <= 768)
+  // (the raw-buffer offsets of the register pass are unsigned 32-bit: a column spans at most
+  // (N - 1) N (N/2) 8 bytes < 4 GiB for N <= 1024)
   return sizeof(R) == 4 && kThreads % W == 0 && N % TT == 0 && (N / TT) % TT == 0 && (TT == 16 || TT == 32) &&
-         N / TT <= 32 && N <= 768;
+         N / TT <= 32 && N <= 1024;
 }
 
 template <typename R, int N, int MODE>
@@ -461,6 +463,11 @@ __device__ __forceinline__ void pencil_transform_reg(const PencilAddr<R>& p, con
 }
 
 // the transform(s) of a staged pencil, in LDS (ends with the LDS holding the result)
+// Passes that run the register transform: every y and z pass up to N = 512; above that only
+// the fused z pass (its LDS slice takes 82-90 KB, one workgroup per CU: the y passes are faster
+// as LDS Stockham passes there, the z pass with its 4 MiB column stride is not)
+template <typename R, int N, int MODE> constexpr bool reg_pass() { return reg_transform<R, N>() && (N <= 512 || MODE == 0); }
+
 template <typename R, int N, int MODE>
 __device__ __forceinline__ void pencil_transform(const PencilAddr<R>& p, const GreenArgs& g, R coef, cpx<R>* lds,
                                                  const cpx<R>* tw, const R* s2, const R* wd, int tid) {
@@ -469,7 +476,7 @@ __device__ __forceinline__ void pencil_transform(const PencilAddr<R>& p, const G
   constexpr int W = pencil_w<R, N>();
   constexpr int SN = W + 1;
   constexpr int NT = kThreads;
-  if constexpr (reg_transform<R, N>()) {
+  if constexpr (reg_pass<R, N, MODE>()) {
     pencil_transform_reg<R, N, MODE>(p, g, coef, lds, tw, s2, wd, tid);
   } else if constexpr (MODE == -1) {
     fft_passes<T, L, 1, -1, W, SN, NT, 1>(lds, tw, tid);
@@ -578,13 +585,15 @@ __device__ __forceinline__ void pencil_reg_body(cpx<R>* H, cpx<R>* Q, const Penc
   pencil_tables<R, N, MODE>(twg, s2g, wdg, tw, s2, wd, tid);
   const PencilAddr<R> p = pencil_addr<R, N>(H, Q, s0, s1, blk);
   // element (n, w) of the pencil at byte n * sn8 + (w * sw) * 8 from its base
-  const int sn8 = (int)((p.contig ? p.stride_n : 1) * (long)sizeof(T));
+  const uint32_t sn8 = (uint32_t)((p.contig ? p.stride_n : 1) * (long)sizeof(T));
   const __amdgpu_buffer_rsrc_t buf = buffer_of(p.base, p.tail * (long)sizeof(T));
-  const int voff = t * sn8 + w * (int)((p.contig ? 1 : p.stride_w) * (long)sizeof(T));
+  const int voff = (int)((uint32_t)t * sn8 + (uint32_t)w * (uint32_t)((p.contig ? 1 : p.stride_w) * (long)sizeof(T)));
+  // byte offset of row r of the column (unsigned: up to 4 GiB at N = 1024)
+  auto roff = [&](int r) { return (int)((uint32_t)r * sn8); };
   auto xrow = [&](int r) { return r * SN + r / TT + w; };
   T v[A];
 #pragma unroll
-  for (int a = 0; a < A; ++a) v[a] = buffer_load(buf, voff, a * TT * sn8);
+  for (int a = 0; a < A; ++a) v[a] = buffer_load(buf, voff, roff(a * TT));
   __syncthreads();  // tables in LDS
   reg_dft<S1, A>(v);
   twiddle_row<S1, N, A, 8>(v, tw, t);
@@ -603,7 +612,7 @@ __device__ __forceinline__ void pencil_reg_body(cpx<R>* H, cpx<R>* Q, const Penc
 #pragma unroll
     for (int j = 0; j < J; ++j)
 #pragma unroll
-      for (int kb = 0; kb < TT; ++kb) buffer_store(buf, voff, (j * TT + A * kb) * sn8, u[j][kb]);
+      for (int kb = 0; kb < TT; ++kb) buffer_store(buf, voff, roff(j * TT + A * kb), u[j][kb]);
   } else {
     // k-space operator on X[k], k = ka + A kb the z mode, then the same steps backwards
     const int kx = p.kx0 + w * p.dkx, ky = p.ky0 + w * p.dky;
@@ -629,7 +638,7 @@ __device__ __forceinline__ void pencil_reg_body(cpx<R>* H, cpx<R>* Q, const Penc
     }
     reg_dft<+1, A>(v);
 #pragma unroll
-    for (int a = 0; a < A; ++a) buffer_store(buf, voff, a * TT * sn8, v[a]);
+    for (int a = 0; a < A; ++a) buffer_store(buf, voff, roff(a * TT), v[a]);
   }
 }
 
@@ -700,7 +709,7 @@ hipError_t launch_pencil(cpx<R>* H, cpx<R>* Q, bool zaxis, const GreenArgs& g, c
   PencilSeg a, b;
   if (zaxis) z_segments<R, N>(a, b);
   else y_segments<R, N>(a, b);
-  if constexpr (reg_transform<R, N>()) {
+  if constexpr (reg_pass<R, N, MODE>()) {
     k_pencil_reg<R, N, MODE><<<a.nblk + b.nblk, kThreads, 0, st>>>(H, Q, a, b, g, (const cpx<R>*)t.tw,
                                                                   (const R*)t.sin2, (const R*)t.wdec);
   } else {

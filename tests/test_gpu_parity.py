@@ -293,9 +293,10 @@ def test_sim_rejects_bad_input():
     s.close()
 
 
-@pytest.mark.parametrize("n,npart", [(256, 32), (512, 32)])
+@pytest.mark.parametrize("n,npart", [(256, 32), (512, 32), (1024, 32)])
 def test_large_grid_fp32_step_matches_fp64(n, npart):
-    # the FP32 passes at these sizes keep each column's DFTs in registers (four-step); the
+    # the FP32 passes at these sizes keep each column's DFTs in registers (four-step; at 1024
+    # a z column spans 4 GiB of the spectrum, the limit of the passes' 32-bit offsets); the
     # FP64 sim runs the LDS Stockham passes.  Same state in, one step: potential and update
     # agree to FP32 accuracy.
     pos = uniform_positions(n, npart ** 3, seed=n)
@@ -307,9 +308,13 @@ def test_large_grid_fp32_step_matches_fp64(n, npart):
     phi32, phi64 = (s.potential(np.float64) for s in sims)
     (x32, v32), (x64, v64) = (s.get_state(np.float64) for s in sims)
     e_phi, e_v = rel(phi32, phi64), rel(v32, v64)
-    print(f"N={n}: phi {e_phi:.2e} accel {e_v:.2e}")
+    # positions: FP32 rounding plus the FP32 force error carried over the step's displacement
+    # (isolated particles on a fine grid are kicked by up to ~25 cells at N = 1024)
+    err = np.abs(periodic_diff(x32, x64, n))
+    step = np.abs(periodic_diff(x64, pos, n))
+    print(f"N={n}: phi {e_phi:.2e} accel {e_v:.2e} position {err.max():.2e} (max step {step.max():.2e})")
     assert e_phi < 1e-5 and e_v < 1e-5
-    assert np.abs(periodic_diff(x32, x64, n)).max() < 1e-4
+    assert np.all(err < 1e-4 + 1e-4 * step)
     for s in sims:
         s.close()
 
