@@ -234,9 +234,21 @@ def ensemble_bench(e, steps=None):
 LARGEST = dict(WORKLOAD, n_cells=1024, n_parts=512, box=500.0, seed=42, steps=500)
 
 
+def loop_steps(a_init, a_end, n_steps):
+    """Steps pmesh.py:56-63 executes for STEPS = n_steps: `while a < A_END - da: ...; a += da`."""
+    da = (a_end - a_init) / n_steps
+    a, k = a_init, 0
+    while a < a_end - da:
+        a += da
+        k += 1
+    return k
+
+
 def largest_bench(w, steps):
-    """A bounded sample (`steps` of the 500) of the largest run, FP32, graph-replayed, timed
-    like the main workload."""
+    """The largest run, FP32, graph-replayed, timed like the main workload: all of its steps
+    (steps < 0) or the first `steps`."""
+    total = loop_steps(w["a_init"], w["a_end"], w["steps"])
+    steps = total if steps < 0 else min(steps, total)
     import initial_conditions as ic
     from pmgpu.sim import PMSim
     sim = PMSim(w["n_cells"], w["n_parts"], 32, omega_m0=w["om"], omega_lambda0=w["ol"],
@@ -257,7 +269,8 @@ def largest_bench(w, steps):
     sim.close()
     ms = dt / steps * 1e3
     return {"workload": "pm_step_512p_1024g: 512^3 particles, 1024^3 grid, box 500 Mpc/h, seed 42, "
-                        f"first {steps} of 500 steps from a = 0.02, device ICs",
+                        + (f"all {total} steps" if steps == total else f"first {steps} of {total} steps")
+                        + f" from a = {w['a_init']}, device ICs",
             "value": n_p * steps / dt, "unit": "particle-updates/s", "ms_per_step": ms,
             "steps": steps, "tiles": list(tiles),
             "step_roofline_frac": step_bytes(n, n_p, 32) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
@@ -425,8 +438,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--largest-steps", type=int, default=20,
-                    help="steps of the 512^3 / 1024^3 run to time (0 skips it)")
+    ap.add_argument("--largest-steps", type=int, default=-1,
+                    help="steps of the 512^3 / 1024^3 run to time (-1: the whole run, 0 skips it)")
     ap.add_argument("--serve-window", type=float, default=60.0,
                     help="seconds of the 600 s serving stream to replay (600 = all of it)")
     ap.add_argument("--ensemble-steps", type=int, default=None)
@@ -514,8 +527,9 @@ def main():
     out["fp32_over_plain_fp64"] = v32 / out["plain_fp64"]["value"]
     out["ensemble"] = None if args.no_ensemble else ensemble_bench(ENSEMBLE, args.ensemble_steps)
     out["serve"] = None if args.no_serve else serve_bench(SERVE, args.serve_window)
-    if args.largest_steps > 0:
-        log(f"largest run: {args.largest_steps} steps of 512^3 particles on 1024^3")
+    if args.largest_steps != 0:
+        log("largest run: " + ("all" if args.largest_steps < 0 else str(args.largest_steps))
+            + " steps of 512^3 particles on 1024^3")
         out["largest"] = largest_bench(LARGEST, args.largest_steps)
     else:
         out["largest"] = None
