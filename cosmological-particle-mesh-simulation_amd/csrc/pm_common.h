@@ -84,6 +84,17 @@ template <typename R> __host__ __device__ __forceinline__ R wrap_disp(R s, R n) 
 }  // namespace pm
 
 // status codes returned by every C-ABI entry point (0 = success)
+// XCD-aware block order.  Workgroups are dealt round-robin to the 8 XCDs of the chip, each with
+// its own L2; this maps dispatch index b of an nblocks-block launch to a logical block so that
+// every XCD works through one contiguous run of logical blocks, and neighbours that share cache
+// lines (adjacent tiles, adjacent column chunks of an FFT pass) use one L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+  const int per = nblocks / kXcds, rem = nblocks % kXcds;
+  const int xcd = b % kXcds, local = b / kXcds;
+  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
 #define PM_OK 0
 #define PM_E_INVALID 1      // bad argument (size, pointer, unsupported grid)
 #define PM_E_HIP 2          // a HIP runtime call failed

@@ -554,7 +554,7 @@ __global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, Penci
   __shared__ cpx<R> tw[N];
   __shared__ R s2[MODE == 0 ? N : 1];
   __shared__ R wd[MODE == 0 ? N : 1];
-  pencil_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, blockIdx.x, lds, tw, s2, wd);
+  pencil_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, xcd_block(blockIdx.x, gridDim.x), lds, tw, s2, wd);
 }
 
 // Register pass: the four-step transform with the columns read from and written to global
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(kThreads) void k_pencil_reg(cpx<R>* H, cpx<R>* Q, P
   __shared__ cpx<R> tw[N];
   __shared__ R s2[MODE == 0 ? N : 1];
   __shared__ R wd[MODE == 0 ? N : 1];
-  pencil_reg_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, blockIdx.x, xb, tw, s2, wd);
+  pencil_reg_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, xcd_block(blockIdx.x, gridDim.x), xb, tw, s2, wd);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -858,11 +858,13 @@ __global__ __launch_bounds__(kThreads) void k_batch_c2r(const BatchDesc* __restr
 #undef PM_CASE
 }
 
+// the pencil passes walk their blocks in XCD order (adjacent column chunks share cache lines);
+// the x passes keep dispatch order, which leaves the spectrum where the next pass reads it
 template <int CLS, int MODE>
 __global__ __launch_bounds__(kThreads) void k_batch_pencil(const BatchDesc* __restrict__ D, long base) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int ST = MODE == 0 ? BS_Z : BS_Y;
-  const long b = base + blockIdx.x;
+  const long b = base + xcd_block(blockIdx.x, gridDim.x);
   const int k = batch_member(D, ST, b);
   const BatchMember& m = D->m[k];
   const long blk = b - D->off[ST][k];

@@ -39,16 +39,9 @@ inline unsigned blocks_for(long n, int b = kT) {
 // index in [-n, 2n) folded into [0, n)
 __device__ __forceinline__ int fold(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-// XCD-aware block -> tile map.  Consecutive workgroups are dealt round-robin to the 8 XCDs,
-// each with its own L2; this hands every XCD one contiguous run of tiles instead, so tiles
-// that share cache lines (x and y neighbours of the potential box, particles straddling a
-// tile face) are processed by the same L2.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
-  const int per = nblocks / kXcds, rem = nblocks % kXcds;
-  const int xcd = b % kXcds, local = b / kXcds;
-  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
-}
+// XCD-aware block -> tile map (tiles that share potential-box lines or straddling particles
+// are processed by one L2)
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) { return xcd_block(b, nblocks); }
 
 // canonical position of a resident particle from its displacement: cell, CIC fractions and
 // lattice point q per axis
