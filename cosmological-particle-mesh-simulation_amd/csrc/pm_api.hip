@@ -532,17 +532,20 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
     StageTimer t(s, ST_R2C);
     PM_HIP((fft_r2c_x<R, R>((const R*)s->rho, H, Q, n, *s->tab, s->stream)));
   }
+  // the density is consumed by the R2C pass and deposited anew after the gather: its buffer
+  // (N^3 reals = N^2 (N/2) complex) is the y / z passes' work array
+  cpx<R>* work = reinterpret_cast<cpx<R>*>(s->rho);
   {
     StageTimer t(s, ST_Y_FWD);
-    PM_HIP(fft_c2c_y<R>(H, Q, n, -1, *s->tab, s->stream));
+    PM_HIP(fft_c2c_y<R>(H, Q, n, -1, *s->tab, s->stream, work));
   }
   {
     StageTimer t(s, ST_Z_GREEN);
-    PM_HIP(fft_z_green<R>(H, Q, n, g, *s->tab, s->stream));
+    PM_HIP(fft_z_green<R>(H, Q, n, g, *s->tab, s->stream, work));
   }
   {
     StageTimer t(s, ST_Y_INV);
-    PM_HIP(fft_c2c_y<R>(H, Q, n, +1, *s->tab, s->stream));
+    PM_HIP(fft_c2c_y<R>(H, Q, n, +1, *s->tab, s->stream, work));
   }
   {
     StageTimer t(s, ST_C2R);

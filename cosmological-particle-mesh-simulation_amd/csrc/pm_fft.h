@@ -242,7 +242,13 @@ struct PencilSeg {
   long stride_w;  // strided segments: distance between columns
   int contig;     // 1: W columns contiguous; 0: the FFT axis is contiguous
   int green_kind; // 0: main H z-pass (kx = col, ky = outer); 1: Q z-pass (kx = N/2, ky = col)
+  // page-tiled spectrum (kPencilTile x kPencilTile blocks of rows; 0: plain layout): outer index
+  // o at (o / kPencilTile) outer_stride_hi + (o % kPencilTile) outer_stride, and row r + T of a
+  // column (T = rows per load step, kPencilTile when tiled) stride_blk elements after row r
+  long outer_stride_hi = 0;
+  long stride_blk = 0;
 };
+constexpr int kPencilTile = 32;
 
 // per-step scalars of a captured step: {green coefficient, kick, drift} per step
 struct StepTable {
@@ -267,12 +273,17 @@ bool fft_size_supported(int n);
 // Host launchers (defined in pm_fft.hip).  All work on stream `st`.
 template <typename R, typename Tin>
 hipError_t fft_r2c_x(const Tin* rho, cpx<R>* H, cpx<R>* Q, int n, const FftTables& t, hipStream_t st);
+// work (optional): an N^2 (N/2)-element array the step's y / z passes may use (at N >= 1024 the
+// forward y pass leaves the spectrum there in a page-tiled layout for the z pass, and the inverse
+// y pass brings it back); both y passes and the z pass of one transform get the same work array
 template <typename R>
-hipError_t fft_c2c_y(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, hipStream_t st);
+hipError_t fft_c2c_y(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, hipStream_t st,
+                     cpx<R>* work = nullptr);
 template <typename R>
 hipError_t fft_c2c_z(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, hipStream_t st);
 template <typename R>
-hipError_t fft_z_green(cpx<R>* H, cpx<R>* Q, int n, const GreenArgs& g, const FftTables& t, hipStream_t st);
+hipError_t fft_z_green(cpx<R>* H, cpx<R>* Q, int n, const GreenArgs& g, const FftTables& t, hipStream_t st,
+                       cpx<R>* work = nullptr);
 template <typename R, typename Tout>
 hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTables& t, hipStream_t st);
 

@@ -268,6 +268,7 @@ __device__ __forceinline__ void twiddle_row(T (&v)[M], const T* tw, int f) {
 template <typename R> struct PencilAddr {
   cpx<R>* base;
   long stride_n, stride_w;
+  long stride_blk;  // from row r to row r + kThreads / W of a column
   int contig;
   int kx0, dkx, ky0, dky;
   long tail;  // elements of the array from base to its end
@@ -285,9 +286,12 @@ __device__ __forceinline__ PencilAddr<R> pencil_addr(cpx<R>* H, cpx<R>* Q, const
   p.contig = s.contig;
   p.stride_n = s.stride_n;
   p.stride_w = s.stride_w;
+  p.stride_blk = s.stride_blk ? s.stride_blk : (long)(kThreads / W) * (s.contig ? s.stride_n : 1);
   if (s.contig) {
     const long o = blk / s.ncol_blk, c = blk - o * s.ncol_blk;
-    p.base = data + o * s.outer_stride + c * W;
+    const long oo = s.outer_stride_hi ? (o / kPencilTile) * s.outer_stride_hi + (o % kPencilTile) * s.outer_stride
+                                      : o * s.outer_stride;
+    p.base = data + oo + c * W;
     if (s.green_kind == 0) {
       p.kx0 = (int)(c * W);
       p.dkx = 1;
@@ -351,7 +355,7 @@ __device__ __forceinline__ void pencil_load(const PencilAddr<R>& p, cpx<R>* lds,
   if (p.contig) {
     const cpx<R>* src = p.base + (tid / W) * p.stride_n + (tid % W);
     cpx<R>* dst = lds + (tid / W) * SN + (tid % W);
-    const long step = (long)(kThreads / W) * p.stride_n;
+    const long step = p.stride_blk;
 #pragma unroll
     for (int i0 = 0; i0 < LPT; i0 += CH) {
       cpx<R> u[CH];
@@ -504,7 +508,7 @@ __device__ __forceinline__ void pencil_store(const PencilAddr<R>& p, const cpx<R
   if (p.contig) {
     cpx<R>* dst = p.base + (tid / W) * p.stride_n + (tid % W);
     const cpx<R>* src = lds + (tid / W) * SN + (tid % W);
-    const long step = (long)(kThreads / W) * p.stride_n;
+    const long step = p.stride_blk;
 #pragma unroll 8
     for (int i = 0; i < LPT; ++i)
       if ((N * W) % kThreads == 0 || i * kThreads + tid < N * W) dst[i * step] = src[i * (kThreads / W) * SN];
@@ -531,30 +535,37 @@ __device__ __forceinline__ void pencil_tables(const cpx<R>* __restrict__ twg, co
 
 // One pencil: fetch, transform, store.  MODE: -1 forward only, +1 inverse only, 0 forward +
 // k-space operator + inverse.
-template <typename R, int N, int MODE>
+// Hout / s0out (optional): the first segment's pencils are written to Hout with the layout s0out
+// instead of back in place (the passes at N >= 1024 move the spectrum between layouts).
+// TAB_LDS: the tables are copied to LDS (tw, s2, wd) or are the global tables themselves.
+template <typename R, int N, int MODE, bool TAB_LDS = true>
 __device__ __forceinline__ void pencil_body(cpx<R>* H, cpx<R>* Q, const PencilSeg& s0, const PencilSeg& s1,
                                             const GreenArgs& g, const cpx<R>* __restrict__ twg,
                                             const R* __restrict__ s2g, const R* __restrict__ wdg, long blk,
-                                            cpx<R>* lds, cpx<R>* tw, R* s2, R* wd) {
+                                            cpx<R>* lds, const cpx<R>* tw, const R* s2, const R* wd,
+                                            cpx<R>* Hout = nullptr, const PencilSeg* s0out = nullptr) {
   const int tid = threadIdx.x;
-  pencil_tables<R, N, MODE>(twg, s2g, wdg, tw, s2, wd, tid);
+  if constexpr (TAB_LDS)
+    pencil_tables<R, N, MODE>(twg, s2g, wdg, const_cast<cpx<R>*>(tw), const_cast<R*>(s2), const_cast<R*>(wd), tid);
   const PencilAddr<R> p = pencil_addr<R, N>(H, Q, s0, s1, blk);
   pencil_load<R, N>(p, lds, tid);
   __syncthreads();
   const R coef = MODE == 0 ? R(g.table ? g.table[3 * (*g.step)] : g.coef) : R(0);
   pencil_transform<R, N, MODE>(p, g, coef, lds, tw, s2, wd, tid);
-  pencil_store<R, N>(p, lds, tid);
+  if (Hout && blk < s0.nblk) pencil_store<R, N>(pencil_addr<R, N>(Hout, Q, *s0out, s1, blk), lds, tid);
+  else pencil_store<R, N>(p, lds, tid);
 }
 
 template <typename R, int N, int MODE>
 __global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, PencilSeg s0, PencilSeg s1, GreenArgs g,
                                                      const cpx<R>* __restrict__ twg, const R* __restrict__ s2g,
-                                                     const R* __restrict__ wdg) {
+                                                     const R* __restrict__ wdg, cpx<R>* Hout, PencilSeg s0out) {
   __shared__ cpx<R> lds[N * (pencil_w<R, N>() + 1)];
-  __shared__ cpx<R> tw[N];
+  __shared__ cpx<R> tw[N];  // (the Stockham passes read the twiddles in every stage: kept in LDS)
   __shared__ R s2[MODE == 0 ? N : 1];
   __shared__ R wd[MODE == 0 ? N : 1];
-  pencil_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, xcd_block(blockIdx.x, gridDim.x), lds, tw, s2, wd);
+  pencil_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, xcd_block(blockIdx.x, gridDim.x), lds, tw, s2, wd, Hout,
+                          &s0out);
 }
 
 // Register pass: the four-step transform with the columns read from and written to global
@@ -568,11 +579,14 @@ template <typename R, int N> constexpr int xslice_elems() {
   return TT * TT * (W + 1) + TT;
 }
 
-template <typename R, int N, int MODE>
+// TAB_LDS: the twiddle and k-space tables are copied to LDS (tw, s2, wd); otherwise tw, s2, wd
+// are the global tables themselves (read through the caches), which keeps the block's LDS to the
+// exchange slice
+template <typename R, int N, int MODE, bool TAB_LDS = true>
 __device__ __forceinline__ void pencil_reg_body(cpx<R>* H, cpx<R>* Q, const PencilSeg& s0, const PencilSeg& s1,
                                                 const GreenArgs& g, const cpx<R>* __restrict__ twg,
                                                 const R* __restrict__ s2g, const R* __restrict__ wdg, long blk,
-                                                cpx<R>* xb, cpx<R>* tw, R* s2, R* wd) {
+                                                cpx<R>* xb, const cpx<R>* tw, const R* s2, const R* wd) {
   using T = cpx<R>;
   constexpr int W = pencil_w<R, N>();
   constexpr int SN = W + 1;
@@ -582,14 +596,16 @@ __device__ __forceinline__ void pencil_reg_body(cpx<R>* H, cpx<R>* Q, const Penc
   constexpr int S1 = MODE == 1 ? +1 : -1;
   const int tid = threadIdx.x;
   const int w = tid % W, t = tid / W;
-  pencil_tables<R, N, MODE>(twg, s2g, wdg, tw, s2, wd, tid);
+  if constexpr (TAB_LDS)
+    pencil_tables<R, N, MODE>(twg, s2g, wdg, const_cast<cpx<R>*>(tw), const_cast<R*>(s2), const_cast<R*>(wd), tid);
   const PencilAddr<R> p = pencil_addr<R, N>(H, Q, s0, s1, blk);
   // element (n, w) of the pencil at byte n * sn8 + (w * sw) * 8 from its base
   const uint32_t sn8 = (uint32_t)((p.contig ? p.stride_n : 1) * (long)sizeof(T));
   const __amdgpu_buffer_rsrc_t buf = buffer_of(p.base, p.tail * (long)sizeof(T));
   const int voff = (int)((uint32_t)t * sn8 + (uint32_t)w * (uint32_t)((p.contig ? 1 : p.stride_w) * (long)sizeof(T)));
-  // byte offset of row r of the column (unsigned: up to 4 GiB at N = 1024)
-  auto roff = [&](int r) { return (int)((uint32_t)r * sn8); };
+  // byte offset of row r (a multiple of TT) of the column (unsigned: up to 4 GiB at N = 1024)
+  const uint32_t blk8 = (uint32_t)(p.stride_blk * (long)sizeof(T));
+  auto roff = [&](int r) { return (int)((uint32_t)(r / TT) * blk8); };
   auto xrow = [&](int r) { return r * SN + r / TT + w; };
   T v[A];
 #pragma unroll
@@ -647,10 +663,18 @@ __global__ __launch_bounds__(kThreads) void k_pencil_reg(cpx<R>* H, cpx<R>* Q, P
                                                          const cpx<R>* __restrict__ twg, const R* __restrict__ s2g,
                                                          const R* __restrict__ wdg) {
   __shared__ cpx<R> xb[xslice_elems<R, N>()];
-  __shared__ cpx<R> tw[N];
-  __shared__ R s2[MODE == 0 ? N : 1];
-  __shared__ R wd[MODE == 0 ? N : 1];
-  pencil_reg_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, xcd_block(blockIdx.x, gridDim.x), xb, tw, s2, wd);
+  // above N = 512 the tables stay in global memory: with them the block's LDS (90 KB at
+  // N = 1024) allows one block per CU, without them two
+  constexpr bool kTabLds = N <= 512;
+  __shared__ cpx<R> tw[kTabLds ? N : 1];
+  __shared__ R s2[kTabLds && MODE == 0 ? N : 1];
+  __shared__ R wd[kTabLds && MODE == 0 ? N : 1];
+  if constexpr (kTabLds)
+    pencil_reg_body<R, N, MODE, true>(H, Q, s0, s1, g, twg, s2g, wdg, xcd_block(blockIdx.x, gridDim.x), xb, tw, s2,
+                                      wd);
+  else
+    pencil_reg_body<R, N, MODE, false>(H, Q, s0, s1, g, twg, s2g, wdg, xcd_block(blockIdx.x, gridDim.x), xb, twg,
+                                       s2g, wdg);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -703,18 +727,65 @@ hipError_t launch_c2r(cpx<R>* H, const cpx<R>* Q, Tout* out, const FftTables& t,
   k_c2r_x<R, Tout, N, RB><<<(long)N * N / RB, kThreads, 0, st>>>(H, Q, out, (const cpx<R>*)t.tw);
   return hipGetLastError();
 }
+// Page-tiled spectrum for N >= 1024 (FP32).  The plain layout puts consecutive z of a column
+// 4 MiB apart, so a z pencil touches one 2 MiB page per row and the pass is bound by address
+// translation (2/3 of its translations missed).  In the tiled layout, blocks of T x T rows
+// (T = kPencilTile) are contiguous: element (z, y, kx) sits at
+//   (z / T) N T M + (y / T) T^2 M + (y % T) T M + (z % T) M + kx        (M = N / 2),
+// so a z column spans N / T pages and a y column 2 N / T.  The step's forward y pass reads the
+// plain spectrum and writes the tiled one to a work array, the z pass runs there, and the
+// inverse y pass writes the plain layout back; the x passes and the Nyquist plane keep theirs.
+template <typename R, int N> constexpr bool tiled_spectrum() {
+  return sizeof(R) == 4 && N >= 1024 && kThreads / pencil_w<R, N>() == kPencilTile;
+}
+template <typename R, int N> PencilSeg y_tiled(PencilSeg a) {  // outer z, rows y
+  constexpr long M = N / 2, T = kPencilTile;
+  a.outer_stride = M;
+  a.outer_stride_hi = (long)N * T * M;
+  a.stride_n = T * M;
+  a.stride_blk = T * T * M;
+  return a;
+}
+template <typename R, int N> PencilSeg z_tiled(PencilSeg a) {  // outer y, rows z
+  constexpr long M = N / 2, T = kPencilTile;
+  a.outer_stride = T * M;
+  a.outer_stride_hi = T * T * M;
+  a.stride_n = M;
+  a.stride_blk = (long)N * T * M;
+  return a;
+}
+
+// work (optional, N^2 (N/2) elements): the tiled work array of the step's y / z passes
 template <typename R, int N, int MODE>
 hipError_t launch_pencil(cpx<R>* H, cpx<R>* Q, bool zaxis, const GreenArgs& g, const FftTables& t,
-                         hipStream_t st) {
+                         hipStream_t st, cpx<R>* work = nullptr) {
   PencilSeg a, b;
   if (zaxis) z_segments<R, N>(a, b);
   else y_segments<R, N>(a, b);
+  const unsigned nb = (unsigned)(a.nblk + b.nblk);
+  const cpx<R>* tw = (const cpx<R>*)t.tw;
+  const R* s2 = (const R*)t.sin2;
+  const R* wd = (const R*)t.wdec;
+  if constexpr (tiled_spectrum<R, N>()) {
+    if (work) {
+      if (zaxis) {  // in place on the tiled array
+        const PencilSeg at = z_tiled<R, N>(a);
+        if constexpr (reg_pass<R, N, MODE>())
+          k_pencil_reg<R, N, MODE><<<nb, kThreads, 0, st>>>(work, Q, at, b, g, tw, s2, wd);
+        else
+          k_pencil<R, N, MODE><<<nb, kThreads, 0, st>>>(work, Q, at, b, g, tw, s2, wd, nullptr, at);
+      } else if (MODE < 0) {  // plain H -> tiled work
+        k_pencil<R, N, MODE><<<nb, kThreads, 0, st>>>(H, Q, a, b, g, tw, s2, wd, work, y_tiled<R, N>(a));
+      } else {  // tiled work -> plain H
+        k_pencil<R, N, MODE><<<nb, kThreads, 0, st>>>(work, Q, y_tiled<R, N>(a), b, g, tw, s2, wd, H, a);
+      }
+      return hipGetLastError();
+    }
+  }
   if constexpr (reg_pass<R, N, MODE>()) {
-    k_pencil_reg<R, N, MODE><<<a.nblk + b.nblk, kThreads, 0, st>>>(H, Q, a, b, g, (const cpx<R>*)t.tw,
-                                                                  (const R*)t.sin2, (const R*)t.wdec);
+    k_pencil_reg<R, N, MODE><<<nb, kThreads, 0, st>>>(H, Q, a, b, g, tw, s2, wd);
   } else {
-    k_pencil<R, N, MODE><<<a.nblk + b.nblk, kThreads, 0, st>>>(H, Q, a, b, g, (const cpx<R>*)t.tw,
-                                                              (const R*)t.sin2, (const R*)t.wdec);
+    k_pencil<R, N, MODE><<<nb, kThreads, 0, st>>>(H, Q, a, b, g, tw, s2, wd, nullptr, a);
   }
   return hipGetLastError();
 }
@@ -751,13 +822,13 @@ hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTabl
   return hipErrorInvalidValue;
 }
 template <typename R>
-hipError_t fft_c2c_y(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, hipStream_t st) {
+hipError_t fft_c2c_y(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, hipStream_t st, cpx<R>* work) {
   GreenArgs g{};
   switch (n) {
 #define PM_CASE(NN)                                                                   \
   case NN:                                                                            \
-    return sign < 0 ? launch_pencil<R, NN, -1>(H, Q, false, g, t, st)                 \
-                    : launch_pencil<R, NN, 1>(H, Q, false, g, t, st);
+    return sign < 0 ? launch_pencil<R, NN, -1>(H, Q, false, g, t, st, work)           \
+                    : launch_pencil<R, NN, 1>(H, Q, false, g, t, st, work);
     PM_FOR_EACH_N(PM_CASE)
 #undef PM_CASE
   }
@@ -777,9 +848,10 @@ hipError_t fft_c2c_z(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, 
   return hipErrorInvalidValue;
 }
 template <typename R>
-hipError_t fft_z_green(cpx<R>* H, cpx<R>* Q, int n, const GreenArgs& g, const FftTables& t, hipStream_t st) {
+hipError_t fft_z_green(cpx<R>* H, cpx<R>* Q, int n, const GreenArgs& g, const FftTables& t, hipStream_t st,
+                       cpx<R>* work) {
   switch (n) {
-#define PM_CASE(NN) case NN: return launch_pencil<R, NN, 0>(H, Q, true, g, t, st);
+#define PM_CASE(NN) case NN: return launch_pencil<R, NN, 0>(H, Q, true, g, t, st, work);
     PM_FOR_EACH_N(PM_CASE)
 #undef PM_CASE
   }
@@ -799,9 +871,10 @@ template hipError_t fft_c2r_x<double, double>(cpx<double>*, const cpx<double>*, 
 template hipError_t fft_c2r_x<double, float>(cpx<double>*, const cpx<double>*, float*, int, const FftTables&,
                                               hipStream_t);
 #define PM_INST(R)                                                                                   \
-  template hipError_t fft_c2c_y<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t);       \
+  template hipError_t fft_c2c_y<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t, cpx<R>*); \
   template hipError_t fft_c2c_z<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t);       \
-  template hipError_t fft_z_green<R>(cpx<R>*, cpx<R>*, int, const GreenArgs&, const FftTables&, hipStream_t);
+  template hipError_t fft_z_green<R>(cpx<R>*, cpx<R>*, int, const GreenArgs&, const FftTables&, hipStream_t,  \
+                                     cpx<R>*);
 PM_INST(float)
 PM_INST(double)
 #undef PM_INST
