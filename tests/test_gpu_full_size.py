@@ -1,0 +1,92 @@
+"""The 256^3-particle / 512^3-grid workload at its full size, checked through properties that do
+not need the float64 oracle to run at that size:
+
+* the FP32 step against the FP64 step from the same state (the FP64 step is pinned to the
+  oracle at the smaller sizes): density, potential and accelerations <= 1e-5 norm-wise;
+* mass conservation of the deposit (sum of rho = N^3, the mean density 1 of pmesh.py:28);
+* momentum conservation of the PM force (sum over particles of the kicks = 0: CIC deposit and
+  CIC gather share their weights and the central difference is odd);
+* bit-exact particle-to-cell assignment of the reference-layout entry point (integrate.py:20);
+* the whole 1000-step run: P(k) of FP32 against FP64 within 0.5 % for k < k_Nyquist/2.
+"""
+import numpy as np
+import pytest
+
+from helpers import ic_params, periodic_diff, rel
+
+pytestmark = pytest.mark.gpu
+
+N, NP, BOX = 512, 256, 250.0
+A0, DA = 0.02, (1.0 - 0.02) / 1000
+
+
+def _sim(prec):
+    from pmgpu.sim import PMSim
+    return PMSim(N, NP, prec, omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+
+
+@pytest.fixture(scope="module")
+def ics():
+    import initial_conditions as ic
+    s = _sim(32)
+    ic.zeldovich_device(s, ic_params(N, NP, box=BOX, seed=7))
+    pos, vel = s.get_state(np.float32)
+    s.close()
+    return pos, vel
+
+
+def test_full_size_step_fp32_matches_fp64(ics):
+    pos, vel = ics
+    sims = [_sim(32), _sim(64)]
+    rho, phi, state0, state1 = [], [], [], []
+    for s in sims:
+        s.set_state(pos, vel)
+        state0.append(s.get_state(np.float64))
+        s.deposit()
+        rho.append(s.density(np.float64))
+        s.step(A0, DA)
+        phi.append(s.potential(np.float64))
+        state1.append(s.get_state(np.float64))
+        s.close()
+    # mass: every particle deposits m = (N / N_PARTS)^3 in total
+    m32, m64 = rho[0].sum() / N ** 3 - 1, rho[1].sum() / N ** 3 - 1
+    (x0, v0), (x0d, v0d) = state0
+    (x32, v32), (x64, v64) = state1
+    acc32, acc64 = v32 - v0, v64 - v0d
+    e_rho, e_phi, e_acc = rel(rho[0], rho[1]), rel(phi[0], phi[1]), rel(acc32, acc64)
+    err = np.abs(periodic_diff(x32, x64, N))
+    step = np.abs(periodic_diff(x64, x0d, N))
+    # momentum: the summed kick per axis against the summed kick magnitude
+    p32 = np.abs(acc32.sum(axis=1)).max() / np.abs(acc32).sum(axis=1).max()
+    p64 = np.abs(acc64.sum(axis=1)).max() / np.abs(acc64).sum(axis=1).max()
+    print(f"512^3/256^3: mass {m32:.1e} / {m64:.1e}; rho {e_rho:.2e} phi {e_phi:.2e} accel {e_acc:.2e}; "
+          f"position {err.max():.2e} (max step {step.max():.2e}); momentum {p32:.1e} / {p64:.1e}")
+    # (measured on MI355X: mass 4.6e-12 / 1.1e-15, momentum 5.8e-9 / 2.2e-16)
+    assert abs(m64) < 1e-12 and abs(m32) < 1e-9
+    assert e_rho < 1e-5 and e_phi < 1e-5 and e_acc < 1e-5
+    assert np.all(err < 1e-4 + 1e-4 * step)
+    assert p64 < 1e-12 and p32 < 1e-6
+
+
+def test_full_size_cell_assignment_bit_exact(ics):
+    from pmgpu import ops
+    pos, _ = ics
+    cells = ops.cell_centers(pos, N)
+    np.testing.assert_array_equal(cells, np.floor(pos).astype(np.int64) % N)
+
+
+def test_full_size_run_power_spectrum_fp32_vs_fp64(ics):
+    pos, vel = ics
+    sims = [_sim(32), _sim(64)]
+    for s in sims:
+        s.set_state(pos, vel)
+        a = s.run(A0, DA, 1000)
+    k, p32, _ = sims[0].power_spectrum(BOX)
+    _, p64, _ = sims[1].power_spectrum(BOX)
+    for s in sims:
+        s.close()
+    sel = np.arange(1, len(k) + 1) <= N // 4   # k < k_Nyquist / 2
+    err = float(np.max(np.abs(p32[sel] / p64[sel] - 1)))
+    print(f"P(k) FP32 vs FP64 after 1000 steps (a = {a:.4f}): max rel err {err:.2e}")
+    assert abs(a - 1.0) < 1e-9
+    assert err < 5e-3
