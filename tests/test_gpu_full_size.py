@@ -68,6 +68,47 @@ def test_full_size_step_fp32_matches_fp64(ics):
     assert p64 < 1e-12 and p32 < 1e-6
 
 
+def _rel_chunked(a, b):
+    """rel() over the leading axis in slabs (grids of 1024^3 values)."""
+    num = den = 0.0
+    for i in range(0, a.shape[0], 64):
+        x, y = np.asarray(a[i:i + 64], np.float64), np.asarray(b[i:i + 64], np.float64)
+        num += float(np.sum((x - y) ** 2))
+        den += float(np.sum(y ** 2))
+    return (num / den) ** 0.5
+
+
+def test_largest_run_size_step_fp32_matches_fp64():
+    # 512^3 particles on a 1024^3 grid (box 500, seed 42): the FP32 step (register FFT passes on
+    # the page-tiled spectrum) against the FP64 step (LDS Stockham passes, plain layout)
+    import initial_conditions as ic
+    from pmgpu.sim import PMSim
+    n, npart = 1024, 512
+    s32 = PMSim(n, npart, 32, omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+    ic.zeldovich_device(s32, ic_params(n, npart, box=500.0, seed=42))
+    pos, vel = s32.get_state(np.float32)
+    out = {}
+    for prec in (32, 64):
+        s = s32 if prec == 32 else PMSim(n, npart, 64, omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+        s.set_state(pos, vel)
+        _, v0 = s.get_state(np.float64)
+        rho = s.density(np.float32)
+        s.step(A0, (1.0 - A0) / 500)
+        phi = s.potential(np.float32)
+        _, v1 = s.get_state(np.float64)
+        v1 -= v0
+        del v0
+        out[prec] = (rho, phi, v1)
+        s.close()
+    mass = float(out[32][0].sum(dtype=np.float64)) / n ** 3 - 1
+    e_rho = _rel_chunked(out[32][0], out[64][0])
+    e_phi = _rel_chunked(out[32][1], out[64][1])
+    e_acc = _rel_chunked(out[32][2], out[64][2])
+    print(f"1024^3/512^3: mass {mass:.1e}; rho {e_rho:.2e} phi {e_phi:.2e} accel {e_acc:.2e}")
+    assert abs(mass) < 1e-6
+    assert e_rho < 1e-5 and e_phi < 1e-5 and e_acc < 1e-5
+
+
 def test_full_size_cell_assignment_bit_exact(ics):
     from pmgpu import ops
     pos, _ = ics
