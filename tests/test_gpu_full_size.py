@@ -7,6 +7,7 @@ not need the float64 oracle to run at that size:
 * momentum conservation of the PM force (sum over particles of the kicks = 0: CIC deposit and
   CIC gather share their weights and the central difference is odd);
 * bit-exact particle-to-cell assignment of the reference-layout entry point (integrate.py:20);
+* the potential of the FP32 and FP64 steps against the same solve done with ROCm's FFT library;
 * the whole 1000-step run: P(k) of FP32 against FP64 within 0.5 % for k < k_Nyquist/2.
 """
 import numpy as np
@@ -107,6 +108,56 @@ def test_largest_run_size_step_fp32_matches_fp64():
     print(f"1024^3/512^3: mass {mass:.1e}; rho {e_rho:.2e} phi {e_phi:.2e} accel {e_acc:.2e}")
     assert abs(mass) < 1e-6
     assert e_rho < 1e-5 and e_phi < 1e-5 and e_acc < 1e-5
+
+
+def _potential_rocfft(rho, a, omega_m0=0.3):
+    """potential.py:7-10 with ROCm's FFT library (hipFFT over rocFFT, float64 D2Z / Z2D) on
+    the GPU: the cross-check of the library's own FFTs.  The k-space factor -3 Om / (8 a) G(k),
+    G = 1 / sum sin^2(k_i / 2), G(0) = 0 (fourier_utils.py:5-16), is applied on the host."""
+    import ctypes as C
+    from pmgpu.device import DeviceArray
+    fft = C.CDLL("/opt/rocm/lib/libhipfft.so")
+    hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    n = rho.shape[0]
+    m = n // 2 + 1
+    d_r = DeviceArray.from_numpy(np.ascontiguousarray(rho, np.float64))
+    d_k = DeviceArray.empty((n, n, m, 2), np.float64)
+    fwd, inv = C.c_void_p(), C.c_void_p()
+    assert fft.hipfftPlan3d(C.byref(fwd), n, n, n, 0x6A) == 0  # HIPFFT_D2Z
+    assert fft.hipfftPlan3d(C.byref(inv), n, n, n, 0x6C) == 0  # HIPFFT_Z2D
+    try:
+        assert fft.hipfftExecD2Z(fwd, C.c_void_p(d_r.ptr), C.c_void_p(d_k.ptr)) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        rk = d_k.numpy().view(np.complex128)[..., 0]
+        s2 = np.sin(np.pi * np.fft.fftfreq(n)) ** 2
+        s2x = np.sin(np.pi * np.fft.rfftfreq(n)) ** 2
+        for z in range(n):  # plane by plane: G of one plane at a time
+            den = s2[z] + s2[:, None] + s2x[None, :]
+            g = np.divide(1.0, den, out=np.zeros_like(den), where=den > 0)
+            rk[z] *= g * (-3.0 * omega_m0 / 8.0 / a)
+        d_k.copy_from(rk.view(np.float64).reshape(n, n, m, 2))
+        del rk
+        assert fft.hipfftExecZ2D(inv, C.c_void_p(d_k.ptr), C.c_void_p(d_r.ptr)) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        return d_r.numpy() / float(n) ** 3
+    finally:
+        fft.hipfftDestroy(fwd)
+        fft.hipfftDestroy(inv)
+
+
+def test_full_size_potential_matches_rocfft(ics):
+    pos, vel = ics
+    for prec, tol in ((32, 1e-5), (64, 1e-12)):
+        s = _sim(prec)
+        s.set_state(pos, vel)
+        rho = s.density(np.float64)
+        s.step(A0, DA)
+        phi = s.potential(np.float64)
+        s.close()
+        ref = _potential_rocfft(rho, A0)
+        e = rel(phi, ref)
+        print(f"512^3 FP{prec} potential vs hipFFT/rocFFT (float64): {e:.2e}")
+        assert e < tol
 
 
 def test_full_size_cell_assignment_bit_exact(ics):
