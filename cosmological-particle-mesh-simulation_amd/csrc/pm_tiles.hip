@@ -485,8 +485,9 @@ __global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, c
 // v_a += da*f(a+da)*g_a (:94) and drifts x_a += da*v_a/(a+da)^2*f(a+da) (:95).  Here the 32
 // distinct phi samples of the 4x4x4 stencil come from the LDS copy of the tile, all three
 // axes are updated in one pass, the displacement is kept in [-N/2, N/2), and the entries of
-// the new position are counted for the next binning.  The potential box is fetched in
-// batches of CH loads per thread before they are written to LDS.
+// the new position are counted for the next binning.  The potential box is fetched into
+// registers (whole z planes per wavefront) while the particle state loads are in flight, then
+// written to LDS.
 // ------------------------------------------------------------------------------------------
 template <int TX, int TY, int TZ> constexpr int gkd_box_cells() { return (TX + 3) * (TY + 3) * (TZ + 3); }
 
