@@ -293,12 +293,12 @@ def test_sim_rejects_bad_input():
     s.close()
 
 
-@pytest.mark.parametrize("n,npart", [(256, 32), (512, 32), (1024, 32)])
+@pytest.mark.parametrize("n,npart", [(256, 32), (384, 48), (512, 32), (768, 48), (1024, 32)])
 def test_large_grid_fp32_step_matches_fp64(n, npart):
-    # the FP32 passes at these sizes keep each column's DFTs in registers (four-step; at 1024
-    # a z column spans 4 GiB of the spectrum, the limit of the passes' 32-bit offsets); the
-    # FP64 sim runs the LDS Stockham passes.  Same state in, one step: potential and update
-    # agree to FP32 accuracy.
+    # the FP32 passes at 256 / 512 / 1024 keep each column's DFTs in registers (four-step; at
+    # 1024 a z column spans 4 GiB of the spectrum, the limit of the passes' 32-bit offsets),
+    # 384 and 768 run LDS Stockham passes in FP32 too; the FP64 sim runs the LDS Stockham
+    # passes.  Same state in, one step: potential and update agree to FP32 accuracy.
     pos = uniform_positions(n, npart ** 3, seed=n)
     vel = np.zeros_like(pos)
     sims = [_sim(n, npart, 32), _sim(n, npart, 64)]
