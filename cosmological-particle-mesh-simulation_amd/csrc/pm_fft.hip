@@ -633,12 +633,25 @@ __device__ __forceinline__ void pencil_reg_body(cpx<R>* H, cpx<R>* Q, const Penc
     // k-space operator on X[k], k = ka + A kb the z mode, then the same steps backwards
     const int kx = p.kx0 + w * p.dkx, ky = p.ky0 + w * p.dky;
     const R coef = R(coef_of(g));
+    if (g.from_grid || g.deconvolve) {
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int kb = 0; kb < TT; ++kb)
+          u[j][kb] = cscale(u[j][kb], green_factor<R>(kx, ky, j * TT + t + A * kb, N, g, coef, s2, wd));
+    } else {  // the plain Green's function: green_factor() without its optional terms
+      const R sxy = s2[kx] + s2[ky];
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int kb = 0; kb < TT; ++kb) {
+          const R den = sxy + s2[j * TT + t + A * kb];
+          u[j][kb] = cscale(u[j][kb], den > R(0) ? coef / den : R(0));
+        }
+    }
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       const int ka = j * TT + t;
-#pragma unroll
-      for (int kb = 0; kb < TT; ++kb)
-        u[j][kb] = cscale(u[j][kb], green_factor<R>(kx, ky, ka + A * kb, N, g, coef, s2, wd));
       reg_dft<+1, TT>(u[j]);
 #pragma unroll
       for (int q = 1; q < TT; ++q) u[j][q] = cmulc(u[j][q], tw[(q * ka) % N]);
