@@ -78,13 +78,17 @@ def make_ics(w):
         return ic.zeldovich_numpy(delta, p)
 
 
-def run_gpu(prec, pos, vel, w, steps, warmup):
+def run_gpu(prec, pos, vel, w, steps, warmup, tiles=None):
     from pmgpu.sim import PMSim
     sim = PMSim(w["n_cells"], w["n_parts"], prec, omega_m0=w["om"], omega_lambda0=w["ol"],
                 omega_k0=w["ok"], h0=w["h"])
     da = (w["a_end"] - w["a_init"]) / w["steps"]
     sim.set_state(pos, vel)
-    sim.tuned = sim.autotune(3)      # tile shape for these particles (set-up, not timed)
+    if tiles:                        # a fixed shape (profiles that must see one kernel variant)
+        sim.set_tiles(*tiles)
+        sim.tuned = (sim.tiles(), None)
+    else:
+        sim.tuned = sim.autotune(3)  # tile shape for these particles (set-up, not timed)
     if warmup:
         sim.run(w["a_init"], da, warmup)
     sim.set_state(pos, vel)          # the timed region starts from the initial conditions
@@ -443,6 +447,8 @@ def main():
     ap.add_argument("--serve-window", type=float, default=60.0,
                     help="seconds of the 600 s serving stream to replay (600 = all of it)")
     ap.add_argument("--ensemble-steps", type=int, default=None)
+    ap.add_argument("--tiles", type=lambda t: tuple(int(x) for x in t.split(",")), default=None,
+                    help="fixed tile shape TX,TY,TZ of the 256^3/512^3 runs instead of the autotuner's")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a known-size kernel for "
                          "calibrating FETCH_SIZE / WRITE_SIZE counters)")
@@ -467,11 +473,11 @@ def main():
     pos, vel = make_ics(w)
 
     log(f"FP32 GPU: {args.warmup} warm-up + {args.steps} timed steps")
-    sim32, dt32, a_end = run_gpu(32, pos, vel, w, args.steps, args.warmup)
+    sim32, dt32, a_end = run_gpu(32, pos, vel, w, args.steps, args.warmup, args.tiles)
     log(f"FP32: {dt32:.3f} s, a_end {a_end:.4f}")
     fp64_steps = args.fp64_steps or args.steps
     log(f"FP64 GPU: {args.warmup} warm-up + {fp64_steps} timed steps")
-    sim64, dt64, _ = run_gpu(64, pos, vel, w, fp64_steps, args.warmup)
+    sim64, dt64, _ = run_gpu(64, pos, vel, w, fp64_steps, args.warmup, args.tiles)
     log(f"FP64: {dt64:.3f} s")
     pk_err = None
     if fp64_steps == args.steps:
