@@ -733,8 +733,10 @@ __device__ __forceinline__ void gkd_body(const GkdState<R>& P, const R* __restri
   gkd_tile<R, TX, TY, TZ, LOCAL_HIST>(P, phi, kick, drift, L, G, entries, counts, tile, beg, end, sl, box, hist);
 }
 
+// FP32 with the LDS histogram: a register budget of 7 workgroups per CU (72 VGPRs, no spills;
+// the kernel waits on dependent loads, so the seventh workgroup pays: 482 -> 470 us at 512^3)
 template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
-__global__ __launch_bounds__(kT) void k_gkd_tiles(GkdState<R> P, const R* __restrict__ phi, R kick, R drift,
+__global__ __launch_bounds__(kT, sizeof(R) == 4 && LOCAL_HIST ? 7 : 1) void k_gkd_tiles(GkdState<R> P, const R* __restrict__ phi, R kick, R drift,
                                                   Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
                                                   uint32_t capacity, StepTable tab) {
