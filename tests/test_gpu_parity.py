@@ -279,6 +279,32 @@ def test_sim_deconvolve_and_fix_flags_change_result():
     assert rel(outs[2] - vel, ratio * (outs[0] - vel)) < 1e-9
 
 
+@pytest.mark.parametrize("n", [256, 512])
+def test_register_pass_green_variants(n):
+    # the FP32 register z pass has three k-space operators: the analytic Green's function, a
+    # user grid G (set_green, symmetrised over k and -k) and the CIC deconvolution; the grid
+    # holding the analytic G must reproduce the analytic step, and the deconvolved FP32 step
+    # the deconvolved FP64 step (LDS Stockham passes)
+    from pmgpu.device import DeviceArray
+    npart = 32
+    pos = uniform_positions(n, npart ** 3, seed=n + 1)
+    vel = np.zeros_like(pos)
+    plain, grid = _sim(n, npart, 32), _sim(n, npart, 32)
+    g = DeviceArray.from_numpy(O.fourier_grid(n).astype(np.float32))
+    grid.set_green(g)
+    dec32, dec64 = _sim(n, npart, 32, cic_deconvolve=True), _sim(n, npart, 64, cic_deconvolve=True)
+    phi = []
+    for s in (plain, grid, dec32, dec64):
+        s.set_state(pos, vel)
+        s.step(0.05, 0.01)
+        phi.append(s.potential(np.float64))
+        s.close()
+    e_grid, e_dec = rel(phi[1], phi[0]), rel(phi[2], phi[3])
+    print(f"N={n}: user grid vs analytic {e_grid:.2e}; deconvolved FP32 vs FP64 {e_dec:.2e}")
+    assert e_grid < 1e-5 and e_dec < 1e-5
+    assert rel(phi[3], phi[0]) > 1e-3  # the deconvolution does change the potential
+
+
 def test_sim_rejects_bad_input():
     from pmgpu.sim import PMSim
     with pytest.raises(RuntimeError, match="supported grid"):
