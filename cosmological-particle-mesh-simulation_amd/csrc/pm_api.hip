@@ -137,7 +137,10 @@ int get_workspace(int n, bool need_phi, Workspace** out) {
 
 int check_grid(int n) {
   if (!fft_size_supported(n))
-    return fail(PM_E_INVALID, "N_CELLS=%d is not a supported grid size (2^a 3^b between 8 and 1024)", n);
+    return fail(PM_E_INVALID,
+                "N_CELLS=%d is not a supported grid size (8, 16, 32, 48, 64, 96, 128, 192, 256, 384, 512, "
+                "768, 1024)",
+                n);
   return PM_OK;
 }
 
@@ -1374,7 +1377,7 @@ int ens_build_graph(pm_ensemble* e) {
 int ens_check_member(const pm_sim* s) {
   if (s->prec != 4) return fail(PM_E_UNSUPPORTED, "ensemble members must be FP32 sims");
   if (!batch_size_supported(s->L.n))
-    return fail(PM_E_UNSUPPORTED, "N_CELLS=%d is not a batched grid size (32..512, 2^a 3^b, multiple of 16)",
+    return fail(PM_E_UNSUPPORTED, "N_CELLS=%d is not a batched grid size (32, 48, 64, 96, 128, 192, 256, 384, 512)",
                 s->L.n);
   if (s->G.tx != 16 || s->G.ty != 16 || s->G.tz != 16)
     return fail(PM_E_UNSUPPORTED, "ensemble members need 16^3 tiles");

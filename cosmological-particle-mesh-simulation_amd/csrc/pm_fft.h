@@ -265,7 +265,7 @@ struct FftTables {
   int prec = 0;           // 4 or 8 (bytes per real)
 };
 
-// Supported grid sizes (2^a 3^b, even); every kernel is instantiated for these.
+// Supported grid sizes (2^a and 3 2^a from 8 to 1024); every kernel is instantiated for these.
 #define PM_FOR_EACH_N(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
 
 bool fft_size_supported(int n);

@@ -164,8 +164,8 @@ PM_API int pm_sim_power_spectrum(pm_sim* sim, double box, int n_bins, double* h_
 
 /* ------------------------------------------- batched ensemble (many sims, one launch/stage) */
 /* No single reference function corresponds: the reference runs one realisation per process
- * (pmesh.py:54-77).  An ensemble advances many FP32 sims, of grid sizes 32..512 (2^a 3^b,
- * multiples of 16) that may differ, with one kernel launch per stage of the step for all of
+ * (pmesh.py:54-77).  An ensemble advances many FP32 sims, of grid sizes 32, 48, 64, 96, 128,
+ * 192, 256, 384 or 512 that may differ, with one kernel launch per stage of the step for all of
  * them (two for the FFT stages when grids above 256 are mixed with smaller ones); the steps
  * replay a captured graph.  Members stay ordinary sims: they keep their buffers, can be stepped
  * alone between ensemble calls and leave the ensemble when destroyed.                         */
