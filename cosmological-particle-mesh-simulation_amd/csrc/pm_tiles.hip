@@ -261,20 +261,29 @@ __global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, con
 constexpr int kScanPer = 16;
 constexpr int kScanChunk = kT * kScanPer;
 
+// Exclusive scan of one value per thread over the block; `total` gets the block's sum.  Each
+// wavefront scans its 64 values with lane shuffles, the wavefront sums meet in LDS (two
+// barriers in all); `sh` may be reused once the call returns.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int off = 1; off < kT; off <<= 1) {
-    const uint32_t t = tid >= off ? sh[tid - off] : 0u;
-    __syncthreads();
-    sh[tid] += t;
-    __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
   }
-  const uint32_t incl = sh[tid];
-  total = sh[kT - 1];
+  if (lane == 63) sh[wv] = x;
   __syncthreads();
-  return incl - v;
+  uint32_t base = 0, sum = 0;
+#pragma unroll
+  for (int k = 0; k < kT / 64; ++k) {
+    const uint32_t w = sh[k];
+    base += k < wv ? w : 0u;
+    sum += w;
+  }
+  total = sum;
+  __syncthreads();
+  return base + x - v;
 }
 
 __device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ counts, long m,
