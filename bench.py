@@ -238,16 +238,6 @@ def ensemble_bench(e, steps=None):
 LARGEST = dict(WORKLOAD, n_cells=1024, n_parts=512, box=500.0, seed=42, steps=500)
 
 
-def loop_steps(a_init, a_end, n_steps):
-    """Steps pmesh.py:56-63 executes for STEPS = n_steps: `while a < A_END - da: ...; a += da`."""
-    da = (a_end - a_init) / n_steps
-    a, k = a_init, 0
-    while a < a_end - da:
-        a += da
-        k += 1
-    return k
-
-
 def largest_bench(w, steps):
     """The largest run, FP32, graph-replayed, timed like the main workload: all of its steps
     (steps < 0) or the first `steps`."""
