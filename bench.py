@@ -54,6 +54,7 @@ def stage_bytes(n, n_p, prec):
         "c2c_y_inv": 2 * half,
         "c2r_x": half + r * cells,                      # read half spectrum, write phi
         "gather_kick_drift": 12 * r * n_p + 4 * n_p + r * cells,  # entries, x, v in/out, phi
+        "deposit_r2c": 3 * r * n_p + 4 * n_p + half,    # entries + displacements in, spectrum out
     }
 
 
@@ -111,9 +112,9 @@ def stage_timing(pos, vel, w, steps, tiles=(0, 0, 0)):
     sim.run(w["a_init"], da, 2)
     sim.set_timing(True)
     sim.run(w["a_init"] + 2 * da, da, steps)
-    t = sim.timing()
+    t = {k: v for k, v in sim.timing().items() if v[1] > 0}   # stages that ran
     sim.close()
-    return {k: ms / max(c, 1) for k, (ms, c) in t.items()}, {k: c for k, (ms, c) in t.items()}
+    return {k: ms / c for k, (ms, c) in t.items()}, {k: c for k, (ms, c) in t.items()}
 
 
 def plain_fp64_bench(pos, vel, w, steps):
@@ -373,7 +374,19 @@ def pk_error(sim32, sim64, box):
     return float(np.max(np.abs(p32[sel] / p64[sel] - 1)))
 
 
-def cpu_baseline(pos, vel, w, max_steps, budget_s):
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pos, vel, w, min_steps, budget_s):
+    """The CPU port on this host's cores: one untimed warm-up step, then at least `min_steps`
+    timed steps (more while the budget lasts); the median step time and the per-stage medians."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pm_oracle as O
     import pm_port
@@ -387,24 +400,75 @@ def cpu_baseline(pos, vel, w, max_steps, budget_s):
     da = (w["a_end"] - w["a_init"]) / w["steps"]
     x, v = pos.copy(), vel.copy()
     a = w["a_init"]
-    stages = {}
-    done = 0
     t0 = time.perf_counter()
-    while done < max_steps:
-        pm_port.step(x, v, fgrid, mass, a, da, w["om"], w["ol"], w["ok"], w["h"], threads, stages)
+    pm_port.step(x, v, fgrid, mass, a, da, w["om"], w["ol"], w["ok"], w["h"], threads)  # warm-up
+    a += da
+    log(f"cpu baseline warm-up step: {time.perf_counter() - t0:.1f} s")
+    steps, stages = [], []
+    t0 = time.perf_counter()
+    while len(steps) < min_steps or time.perf_counter() - t0 < budget_s * 0.5:
+        st = {}
+        ts = time.perf_counter()
+        pm_port.step(x, v, fgrid, mass, a, da, w["om"], w["ol"], w["ok"], w["h"], threads, st)
+        steps.append(time.perf_counter() - ts)
+        stages.append(st)
         a += da
-        done += 1
-        log(f"cpu baseline step {done}: {time.perf_counter() - t0:.1f} s")
+        log(f"cpu baseline step {len(steps)}: {steps[-1]:.2f} s")
         if time.perf_counter() - t0 > budget_s:
             break
-    dt = time.perf_counter() - t0
+    med = float(np.median(steps))
     n_p = w["n_parts"] ** 3
-    return {"value": n_p * done / dt, "unit": "particle-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{done} full PM steps of the same 256^3/512^3 workload from the same ICs "
-                      f"(C/OpenMP port of density.py/integrate.py + scipy.fft complex128 c2c), "
-                      f"{dt:.1f} s",
-            "stage_s_per_step": {k: s / done for k, s in stages.items()}}
+    return {"value": n_p / med, "unit": "particle-updates/s", "cores": threads,
+            "kind": "port", "cpu": cpu_model(),
+            "sample": f"median of {len(steps)} PM steps after 1 warm-up step, the same "
+                      f"256^3/512^3 workload from the same initial conditions (C/OpenMP port of "
+                      f"density.py/integrate.py + scipy.fft complex128 c2c standing in for FFTW)",
+            "median_s_per_step": med, "min_s_per_step": float(np.min(steps)),
+            "stage_median_s": {k: float(np.median([st[k] for st in stages])) for k in stages[0]}}
+
+
+def copy_ceiling(gib=2, reps=5):
+    """Device-to-device copy of `gib` GiB (hipMemcpy, a copy kernel on this device): the
+    read + write bandwidth a streaming kernel can reach, measured in this run."""
+    from pmgpu import _lib
+    from pmgpu.device import DeviceArray
+    nbytes = gib << 30
+    src = DeviceArray.zeros((nbytes // 4,), np.float32)
+    dst = DeviceArray.empty((nbytes // 4,), np.float32)
+    _lib.call("pm_memcpy_dtod", dst.ptr, src.ptr, nbytes)     # warm-up
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _lib.call("pm_memcpy_dtod", dst.ptr, src.ptr, nbytes)
+        best = min(best, time.perf_counter() - t0)
+    del src, dst
+    gbs = 2 * nbytes / best / 1e9
+    return {"bytes_per_copy": 2 * nbytes, "GBs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS,
+            "how": f"hipMemcpy device-to-device of {gib} GiB, best of {reps}, read + write bytes"}
+
+
+def late_time(pos, vel, w, steps, tiles):
+    """The same number of FP32 steps timed at the end of the run (a -> 1), where clustering is
+    strongest: the run is advanced untimed to `steps` before its end, then timed."""
+    from pmgpu.sim import PMSim
+    sim = PMSim(w["n_cells"], w["n_parts"], 32, omega_m0=w["om"], omega_lambda0=w["ol"],
+                omega_k0=w["ok"], h0=w["h"])
+    sim.set_tiles(*tiles)
+    sim.set_state(pos, vel)
+    da = (w["a_end"] - w["a_init"]) / w["steps"]
+    total = loop_steps(w["a_init"], w["a_end"], w["steps"])
+    a = sim.run(w["a_init"], da, total - steps)
+    sim.sync()
+    t0 = time.perf_counter()
+    a_end = sim.run(a, da, steps)
+    sim.sync()
+    dt = time.perf_counter() - t0
+    sim.close()
+    ms = dt / steps * 1e3
+    return {"steps": steps, "a_start": a, "a_end": a_end, "ms_per_step": ms,
+            "value": w["n_parts"] ** 3 / (ms * 1e-3),
+            "step_roofline_frac": step_bytes(w["n_cells"], w["n_parts"] ** 3, 32) / (ms * 1e-3) / 1e9
+            / HBM_PEAK_GBS}
 
 
 def traffic_from_profiles(kernel):
@@ -427,7 +491,7 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=50)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
@@ -518,6 +582,15 @@ def main():
                      "avg_launch_ms": stage_ms[dom], "launches": stage_calls[dom]},
         "vs_reference_readme_cpu": v32 / 4.66e6,
     }
+    log("copy ceiling")
+    out["copy_ceiling"] = copy_ceiling()
+    out["roofline"]["frac_of_copy_ceiling"] = ach / out["copy_ceiling"]["GBs"]
+    out["step_roofline"]["frac_of_copy_ceiling"] = (out["step_roofline"]["achieved_GBs"]
+                                                    / out["copy_ceiling"]["GBs"])
+    total = loop_steps(w["a_init"], w["a_end"], w["steps"])
+    if args.steps < total:
+        log(f"late-time window: the last {args.steps} steps of the run")
+        out["late_time"] = late_time(pos, vel, w, args.steps, tiles32)
     log("plain FP64 GPU path (reference-layout entry points)")
     out["plain_fp64"] = plain_fp64_bench(pos, vel, w, args.plain_steps)
     out["fp32_over_plain_fp64"] = v32 / out["plain_fp64"]["value"]

@@ -386,8 +386,13 @@ struct pm_sim {
   void* vb[2] = {nullptr, nullptr};
   uint32_t* idb[2] = {nullptr, nullptr};
   int cur = 0;
-  void* rho = nullptr;
-  void* H = nullptr;  // half spectrum; the potential is written over it
+  // two N^3 grids: the density slot grid[gb] and the spectrum slot grid[gb ^ 1].  A step
+  // transforms the density into the spectrum slot and writes the potential over it there.  With
+  // the fused deposit + R2C the next step's x spectrum is written into the density slot instead
+  // (spec_valid); that step runs its passes there and swaps the slots, so the potential again
+  // ends up in the spectrum slot.
+  void* grid[2] = {nullptr, nullptr};
+  int gb = 0;
   void* Q = nullptr;
   const float* fgrid = nullptr;  // optional caller-owned Green's function grid
   FftTables* tab = nullptr;
@@ -395,11 +400,14 @@ struct pm_sim {
   Bins B{};
   hipStream_t stream = nullptr;
   bool has_state = false, rho_valid = false, phi_valid = false, bins_valid = false;
+  bool spec_valid = false;      // the density slot holds the x spectrum of the current density
+  uint32_t* h_health = nullptr; // pinned host copy of the health counters (B.health on the device)
   // graph replay of consecutive steps: per-step scalars in a device table read at *d_step
   double* d_table = nullptr;   // kTableSteps x {green coefficient, kick, drift}
   int* d_step = nullptr;
-  // graph[c]: kGraphSteps (an even count) steps captured starting from state buffer c
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // graph[c][g]: kGraphSteps (an even count) steps captured starting from state buffer c with
+  // density slot g
+  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   bool timing = false;
   std::vector<PendingEvent> pending;
   double stage_ms[ST_COUNT] = {};
@@ -424,14 +432,50 @@ size_t bins_bytes(int n, int np_side, int prec) {
   return 4 * (3 * 2 * (size_t)g.ntiles + 1 + (size_t)scan_blocks(g) + 1) + 4 * 8 * np;
 }
 
+// per-sim allocations besides the grids, particles and bins: the step table of graph replays,
+// the captured graphs and the health counters (a generous bound)
+constexpr size_t kSimOverhead = 1u << 20;
+
 size_t sim_bytes(int n, int np_side, int prec) {
   const size_t nn = (size_t)n, np = (size_t)np_side * np_side * np_side;
   return 2 * (6 * np * prec + 4 * np) /*two state buffers*/ + nn * nn * nn * prec /*rho*/ +
-         nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/ + bins_bytes(n, np_side, prec);
+         nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/ + bins_bytes(n, np_side, prec) + kSimOverhead;
 }
 
 template <typename R> PState<R> state_buf(const pm_sim* s, int k) { return {(R*)s->sb[k], (R*)s->vb[k], s->idb[k]}; }
 template <typename R> PState<R> cur_state(const pm_sim* s) { return state_buf<R>(s, s->cur); }
+void* rho_slot(const pm_sim* s) { return s->grid[s->gb]; }
+// fixed-point units per unit mass of the deterministic deposit (0: floating-point sums)
+double det_fix(const pm_sim* s) {
+  return (s->flags & PM_FLAG_DETERMINISTIC) ? deterministic_fix(s->mass * (double)s->L.np, s->L.np) : 0.0;
+}
+void* spec_slot(const pm_sim* s) { return s->grid[s->gb ^ 1]; }
+
+// the step's deposit and R2C x pass run as one row-block kernel (FP32, tiles of 8 x 8 cells in
+// y and z, grid sizes with an instantiation) unless the sim asks for the separate passes
+bool fused_deposit(const pm_sim* s) {
+  return s->prec == 4 && !(s->flags & (PM_FLAG_SEPARATE_DEPOSIT | PM_FLAG_DETERMINISTIC)) &&
+         deposit_r2c_supported(s->L.n, s->G);
+}
+
+// Health counters: enqueue their copy to the pinned host mirror on `st` before the caller's
+// synchronisation, then judge them after it.  A problem clears the counters and fails.
+int health_fetch(pm_sim* s, hipStream_t st) {
+  PM_HIP(hipMemcpyAsync(s->h_health, s->B.health, sizeof(uint32_t) * PM_HEALTH_COUNT, hipMemcpyDeviceToHost, st));
+  return PM_OK;
+}
+int health_verdict(pm_sim* s, hipStream_t st) {
+  const uint32_t* h = s->h_health;
+  if (!(h[PM_HEALTH_NONFINITE] | h[PM_HEALTH_BINS] | h[PM_HEALTH_OVERFLOW])) return PM_OK;
+  const uint32_t nf = h[PM_HEALTH_NONFINITE], nb = h[PM_HEALTH_BINS], no = h[PM_HEALTH_OVERFLOW];
+  PM_HIP(hipMemsetAsync(s->B.health, 0, sizeof(uint32_t) * PM_HEALTH_COUNT, st));
+  PM_HIP(hipStreamSynchronize(st));
+  std::memset(s->h_health, 0, sizeof(uint32_t) * PM_HEALTH_COUNT);
+  return fail(PM_E_CORRUPT,
+              "particle state is corrupt: %u non-finite positions or velocities, %u bin entries outside "
+              "their tile, %u entries without a place in their bin",
+              nf, nb, no);
+}
 
 // records HIP events around a stage on the owner's stream when its timing is on (the owner is
 // a sim or an ensemble)
@@ -486,8 +530,22 @@ template <typename R> int sim_deposit_impl(pm_sim* s, int* step = nullptr) {
   if (!s->bins_valid)
     if (int rc = sim_bin_impl<R>(s)) return rc;
   StageTimer t(s, ST_DEPOSIT);
-  PM_HIP(deposit_tiles<R>(cur_state<R>(s), (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream, step));
+  PM_HIP(deposit_tiles<R>(cur_state<R>(s), (R*)rho_slot(s), s->mass, s->L, s->G, s->B, s->stream, step,
+                          det_fix(s)));
   s->rho_valid = true;
+  s->spec_valid = false;
+  return PM_OK;
+}
+
+// fused deposit + R2C x pass into the density slot (FP32 only; fused_deposit(s) must hold)
+int sim_deposit_spec_impl(pm_sim* s, int* step = nullptr) {
+  if (!s->bins_valid)
+    if (int rc = sim_bin_impl<float>(s)) return rc;
+  StageTimer t(s, ST_DEPOSIT_R2C);
+  PM_HIP(deposit_r2c(cur_state<float>(s), (float)s->mass, s->L, s->G, s->B, *s->tab, (cpx<float>*)rho_slot(s),
+                     (cpx<float>*)s->Q, s->stream, step));
+  s->spec_valid = true;
+  s->rho_valid = false;
   return PM_OK;
 }
 
@@ -513,11 +571,15 @@ StepScalars step_scalars(const pm_sim* s, double a, double da) {
 // for every step and can be captured once into a graph.
 template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool table = false) {
   const int n = s->L.n;
-  if (!s->rho_valid || !s->bins_valid) {
+  const bool fused = sizeof(R) == 4 && fused_deposit(s);
+  if (!s->spec_valid && (!s->rho_valid || !s->bins_valid)) {
     if (table) return fail(PM_E_STATE, "a captured step needs a valid density");
-    if (int rc = sim_deposit_impl<R>(s)) return rc;
+    if (fused) {
+      if (int rc = sim_deposit_spec_impl(s)) return rc;
+    } else if (int rc = sim_deposit_impl<R>(s)) {
+      return rc;
+    }
   }
-  cpx<R>* H = (cpx<R>*)s->H;
   cpx<R>* Q = (cpx<R>*)s->Q;
   const StepScalars sc = step_scalars(s, a, da);
   GreenArgs g{};
@@ -531,13 +593,15 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
     g.from_grid = 1;
     g.fgrid = s->fgrid;
   }
-  {
+  // the spectrum: left in the density slot by the fused deposit, or the R2C pass of the density
+  const bool in_rho_slot = s->spec_valid;
+  cpx<R>* H = (cpx<R>*)(in_rho_slot ? rho_slot(s) : spec_slot(s));
+  if (!in_rho_slot) {
     StageTimer t(s, ST_R2C);
-    PM_HIP((fft_r2c_x<R, R>((const R*)s->rho, H, Q, n, *s->tab, s->stream)));
+    PM_HIP((fft_r2c_x<R, R>((const R*)rho_slot(s), H, Q, n, *s->tab, s->stream)));
   }
-  // the density is consumed by the R2C pass and deposited anew after the gather: its buffer
-  // (N^3 reals = N^2 (N/2) complex) is the y / z passes' work array
-  cpx<R>* work = reinterpret_cast<cpx<R>*>(s->rho);
+  // the other slot is the y / z passes' work array (the density it may hold has been consumed)
+  cpx<R>* work = reinterpret_cast<cpx<R>*>(in_rho_slot ? spec_slot(s) : rho_slot(s));
   {
     StageTimer t(s, ST_Y_FWD);
     PM_HIP(fft_c2c_y<R>(H, Q, n, -1, *s->tab, s->stream, work));
@@ -552,9 +616,11 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
   }
   {
     StageTimer t(s, ST_C2R);
-    PM_HIP((fft_c2r_x<R, R>(H, Q, (R*)s->H, n, *s->tab, s->stream)));
+    PM_HIP((fft_c2r_x<R, R>(H, Q, (R*)H, n, *s->tab, s->stream)));
   }
+  if (in_rho_slot) s->gb ^= 1;  // the potential now sits in the spectrum slot
   s->phi_valid = true;
+  s->spec_valid = s->rho_valid = false;
   StepTable tab;
   if (table) {
     tab.table = s->d_table;
@@ -562,19 +628,19 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
   }
   {
     StageTimer t(s, ST_GKD);
-    PM_HIP(gkd_tiles<R>(cur_state<R>(s), state_buf<R>(s, s->cur ^ 1), (const R*)s->H, sc.kick, sc.drift, s->L, s->G,
-                        s->B, tab, s->stream));
+    PM_HIP(gkd_tiles<R>(cur_state<R>(s), state_buf<R>(s, s->cur ^ 1), (const R*)spec_slot(s), sc.kick, sc.drift,
+                        s->L, s->G, s->B, tab, s->stream));
   }
   s->cur ^= 1;
-  s->rho_valid = false;
   {
     // the gather counted the entries of the new positions: finish their binning
     StageTimer t(s, ST_BIN);
     PM_HIP(bins_scan(s->G, s->B, s->stream));
     PM_HIP(bins_scatter<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream));
   }
-  if (int rc = sim_deposit_impl<R>(s, table ? s->d_step : nullptr)) return rc;
-  return PM_OK;
+  int* step = table ? s->d_step : nullptr;
+  if (fused) return sim_deposit_spec_impl(s, step);
+  return sim_deposit_impl<R>(s, step);
 }
 
 constexpr int kTableSteps = 4096;  // steps per uploaded scalar table
@@ -585,8 +651,8 @@ static_assert(kGraphSteps % 2 == 0 && kTableSteps % 2 == 0, "a graph replay must
 // capture kGraphSteps table-driven steps of this sim, starting from the current state buffer,
 // into an executable graph (once per buffer)
 template <typename R> int sim_build_graph(pm_sim* s) {
-  const int c0 = s->cur;
-  if (s->graph[c0]) return PM_OK;
+  const int c0 = s->cur, g0 = s->gb;
+  if (s->graph[c0][g0]) return PM_OK;
   if (!s->d_table) {
     PM_HIP(hipMalloc(&s->d_table, sizeof(double) * 3 * kTableSteps));
     PM_HIP(hipMalloc(&s->d_step, sizeof(int)));
@@ -594,18 +660,25 @@ template <typename R> int sim_build_graph(pm_sim* s) {
   hipGraph_t graph = nullptr;
   PM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = PM_OK;
+  const bool rho_v = s->rho_valid, spec_v = s->spec_valid, phi_v = s->phi_valid;
   for (int k = 0; k < kGraphSteps && rc == PM_OK; ++k) rc = sim_step_impl<R>(s, 1.0, 0.0, true);
   hipError_t e = hipStreamEndCapture(s->stream, &graph);
-  s->cur = c0;  // capture enqueued nothing; an even step count ends on the same buffer anyway
+  // capture enqueued nothing: restore the host view (an even step count ends on the same state
+  // buffer and density slot anyway)
+  s->cur = c0;
+  s->gb = g0;
+  s->rho_valid = rho_v;
+  s->spec_valid = spec_v;
+  s->phi_valid = phi_v;
   if (rc != PM_OK) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc;
   }
   if (e != hipSuccess) return fail(PM_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&s->graph[c0], graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&s->graph[c0][g0], graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) {
-    s->graph[c0] = nullptr;
+    s->graph[c0][g0] = nullptr;
     return fail(PM_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
   }
   return PM_OK;
@@ -613,8 +686,14 @@ template <typename R> int sim_build_graph(pm_sim* s) {
 
 // n steps from a through replays of the captured graph (remainder steps table-driven, direct)
 template <typename R> int sim_run_graph(pm_sim* s, double& a, double da, int n_steps) {
-  if (!s->rho_valid || !s->bins_valid)
+  // a captured step starts from the state its predecessor leaves: the fused deposit's spectrum,
+  // or a density and valid bins
+  if (sizeof(R) == 4 && fused_deposit(s)) {
+    if (!s->spec_valid)
+      if (int rc = sim_deposit_spec_impl(s)) return rc;
+  } else if (!s->rho_valid || !s->bins_valid) {
     if (int rc = sim_deposit_impl<R>(s)) return rc;
+  }
   if (int rc = sim_build_graph<R>(s)) return rc;
   std::vector<double> host(3 * (size_t)kTableSteps);
   int done = 0;
@@ -632,22 +711,29 @@ template <typename R> int sim_run_graph(pm_sim* s, double& a, double da, int n_s
     PM_HIP(hipMemcpyAsync(s->d_table, host.data(), sizeof(double) * 3 * chunk, hipMemcpyHostToDevice, s->stream));
     PM_HIP(hipMemcpyAsync(s->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, s->stream));
     int k = 0;
-    for (; k + kGraphSteps <= chunk; k += kGraphSteps) PM_HIP(hipGraphLaunch(s->graph[s->cur], s->stream));
+    for (; k + kGraphSteps <= chunk; k += kGraphSteps) {
+      PM_HIP(hipGraphLaunch(s->graph[s->cur][s->gb], s->stream));
+      s->phi_valid = true;
+    }
     for (; k < chunk; ++k)
       if (int rc = sim_step_impl<R>(s, 1.0, 0.0, true)) return rc;
     // the host buffers must outlive the asynchronous copies above
+    if (int rc = health_fetch(s, s->stream)) return rc;
     PM_HIP(hipStreamSynchronize(s->stream));
+    if (int rc = health_verdict(s, s->stream)) return rc;
     done += chunk;
   }
-  s->rho_valid = s->bins_valid = s->phi_valid = true;
+  // replays end where they started (even step counts); the direct remainder steps kept the flags
+  s->bins_valid = true;
   return PM_OK;
 }
 
 void sim_drop_graph(pm_sim* s) {
-  for (hipGraphExec_t& g : s->graph) {
-    if (g) (void)hipGraphExecDestroy(g);
-    g = nullptr;
-  }
+  for (auto& row : s->graph)
+    for (hipGraphExec_t& g : row) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
 }
 
 // use tile shape g from now on (the captured graph holds the old shape's launches)
@@ -657,7 +743,7 @@ int sim_use_tiles(pm_sim* s, const TileGeom& g) {
   if (g.tx != s->G.tx || g.ty != s->G.ty || g.tz != s->G.tz) {
     s->G = g;
     s->bins_valid = false;
-    s->rho_valid = false;
+    s->rho_valid = s->spec_valid = false;
     sim_drop_graph(s);
   }
   return PM_OK;
@@ -670,7 +756,7 @@ template <typename R> int sim_autotune_impl(pm_sim* s, int reps, double* best_ms
   const int n = s->L.n;
   const size_t nn = (size_t)n;
   // the no-op gather reads the potential buffer: give it zeros (finite forces times zero)
-  PM_HIP(hipMemsetAsync(s->H, 0, nn * nn * nn * s->prec, s->stream));
+  PM_HIP(hipMemsetAsync(spec_slot(s), 0, nn * nn * nn * s->prec, s->stream));
   s->phi_valid = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   PM_HIP(hipEventCreate(&e0));
@@ -685,9 +771,10 @@ template <typename R> int sim_autotune_impl(pm_sim* s, int reps, double* best_ms
     hipError_t e = hipSuccess;
     for (int r = -1; r < reps && e == hipSuccess; ++r) {  // r = -1: warm-up
       if (r == 0) e = hipEventRecord(e0, s->stream);
-      if (!e) e = deposit_tiles<R>(cur_state<R>(s), (R*)s->rho, s->mass, s->L, s->G, s->B, s->stream);
+      if (!e) e = deposit_tiles<R>(cur_state<R>(s), (R*)rho_slot(s), s->mass, s->L, s->G, s->B, s->stream, nullptr,
+                                   det_fix(s));
       if (!e) {
-        e = gkd_tiles<R>(cur_state<R>(s), state_buf<R>(s, s->cur ^ 1), (const R*)s->H, 0.0, 0.0, s->L, s->G, s->B,
+        e = gkd_tiles<R>(cur_state<R>(s), state_buf<R>(s, s->cur ^ 1), (const R*)spec_slot(s), 0.0, 0.0, s->L, s->G, s->B,
                          StepTable{}, s->stream);
         if (!e) s->cur ^= 1;
       }
@@ -711,7 +798,7 @@ template <typename R> int sim_autotune_impl(pm_sim* s, int reps, double* best_ms
   (void)hipEventDestroy(e1);
   s->G = bg;
   s->bins_valid = false;
-  s->rho_valid = false;
+  s->rho_valid = s->spec_valid = false;
   sim_drop_graph(s);
   if (rc) return rc;
   if (best_ms) *best_ms = best;
@@ -793,8 +880,8 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
     ok = ok && hipMalloc(&s->sb[k], 3 * (size_t)s->L.np * pr) == hipSuccess &&
          hipMalloc(&s->vb[k], 3 * (size_t)s->L.np * pr) == hipSuccess &&
          hipMalloc(&s->idb[k], (size_t)s->L.np * 4) == hipSuccess;
-  if (!ok || hipMalloc(&s->rho, nn * nn * nn * pr) != hipSuccess || hipMalloc(&s->H, nn * nn * nn * pr) != hipSuccess ||
-      hipMalloc(&s->Q, nn * nn * 2 * pr) != hipSuccess)
+  if (!ok || hipMalloc(&s->grid[0], nn * nn * nn * pr) != hipSuccess ||
+      hipMalloc(&s->grid[1], nn * nn * nn * pr) != hipSuccess || hipMalloc(&s->Q, nn * nn * 2 * pr) != hipSuccess)
     return cleanup(fail(PM_E_NOMEM, "device allocation of %zu bytes failed", sim_bytes(n_cells, n_parts_side, s->prec)));
   s->G = make_tiles(n_cells, s->prec);
   const TileGeom gmax = max_tiles(n_cells, s->prec);
@@ -803,8 +890,11 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
   s->B.capacity = (uint32_t)(8 * (size_t)s->L.np);
   if (hipMalloc(&s->B.counts, m * 4) != hipSuccess || hipMalloc(&s->B.offsets, (m + 1) * 4) != hipSuccess ||
       hipMalloc(&s->B.cursors, m * 4) != hipSuccess || hipMalloc(&s->B.entries, 8 * (size_t)s->L.np * 4) != hipSuccess ||
-      hipMalloc(&s->B.block_sums, ((size_t)scan_blocks(gmax) + 1) * 4) != hipSuccess)
+      hipMalloc(&s->B.block_sums, ((size_t)scan_blocks(gmax) + 1) * 4) != hipSuccess ||
+      hipMalloc(&s->B.health, sizeof(uint32_t) * PM_HEALTH_COUNT) != hipSuccess ||
+      hipHostMalloc(&s->h_health, sizeof(uint32_t) * PM_HEALTH_COUNT, hipHostMallocDefault) != hipSuccess)
     return cleanup(fail(PM_E_NOMEM, "device allocation of the tile bins failed"));
+  std::memset(s->h_health, 0, sizeof(uint32_t) * PM_HEALTH_COUNT);
   // every entry holds a valid slot from the start, counters start at zero
   s->B.done = s->B.block_sums + scan_blocks(gmax);
   if (hipMemset(s->B.entries, 0, 8 * (size_t)s->L.np * 4) != hipSuccess ||
@@ -812,7 +902,8 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
       hipMemset(s->idb[1], 0, (size_t)s->L.np * 4) != hipSuccess ||
       hipMemset(s->B.done, 0, 4) != hipSuccess ||
       hipMemset(s->B.counts, 0, m * 4) != hipSuccess || hipMemset(s->B.cursors, 0, m * 4) != hipSuccess ||
-      hipMemset(s->B.offsets, 0, (m + 1) * 4) != hipSuccess)
+      hipMemset(s->B.offsets, 0, (m + 1) * 4) != hipSuccess ||
+      hipMemset(s->B.health, 0, sizeof(uint32_t) * PM_HEALTH_COUNT) != hipSuccess)
     return cleanup(fail(PM_E_HIP, "clearing the tile bins failed"));
   if (hipStreamCreate(&s->stream) != hipSuccess) return cleanup(fail(PM_E_HIP, "hipStreamCreate failed"));
   *out = s;
@@ -832,8 +923,10 @@ PM_API int pm_sim_destroy(pm_sim* s) {
     (void)hipFree(s->vb[k]);
     (void)hipFree(s->idb[k]);
   }
-  (void)hipFree(s->rho);
-  (void)hipFree(s->H);
+  (void)hipFree(s->grid[0]);
+  (void)hipFree(s->grid[1]);
+  (void)hipFree(s->B.health);
+  if (s->h_health) (void)hipHostFree(s->h_health);
   (void)hipFree(s->Q);
   (void)hipFree(s->B.counts);
   (void)hipFree(s->B.offsets);
@@ -861,7 +954,7 @@ PM_API int pm_sim_set_mass(pm_sim* s, double mass) {
   if (!s) return fail(PM_E_INVALID, "null sim");
   if (!(mass > 0.0)) return fail(PM_E_INVALID, "mass must be positive");
   if (mass != s->mass) {
-    s->rho_valid = false;
+    s->rho_valid = s->spec_valid = false;
     sim_drop_graph(s);  // the captured deposit holds the old mass
   }
   s->mass = mass;
@@ -880,7 +973,7 @@ PM_API int pm_sim_set_density(pm_sim* s, const void* rho, int dtype_bytes, int o
   if (dtype_bytes != 4 && dtype_bytes != 8) return fail(PM_E_INVALID, "dtype_bytes must be 4 or 8");
   const size_t cnt = (size_t)s->L.n * s->L.n * s->L.n;
   if (dtype_bytes == s->prec) {
-    PM_HIP(hipMemcpyAsync(s->rho, rho, cnt * dtype_bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+    PM_HIP(hipMemcpyAsync(rho_slot(s), rho, cnt * dtype_bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                           s->stream));
   } else {
     const void* src = rho;
@@ -893,14 +986,15 @@ PM_API int pm_sim_set_density(pm_sim* s, const void* rho, int dtype_bytes, int o
       }
       src = tmp;
     }
-    hipError_t e = s->prec == 4 ? convert_grid<double, float>((const double*)src, (float*)s->rho, cnt, s->stream)
-                                : convert_grid<float, double>((const float*)src, (double*)s->rho, cnt, s->stream);
+    hipError_t e = s->prec == 4 ? convert_grid<double, float>((const double*)src, (float*)rho_slot(s), cnt, s->stream)
+                                : convert_grid<float, double>((const float*)src, (double*)rho_slot(s), cnt, s->stream);
     hipError_t es = hipStreamSynchronize(s->stream);
     if (tmp) (void)hipFree(tmp);
     if (e != hipSuccess || es != hipSuccess) return fail(PM_E_HIP, "density upload failed");
   }
   PM_HIP(hipStreamSynchronize(s->stream));
   s->rho_valid = true;
+  s->spec_valid = false;
   return PM_OK;
 }
 
@@ -923,23 +1017,32 @@ PM_API int pm_sim_set_state(pm_sim* s, const void* pos, const void* vel, int dty
     }
   }
   hipError_t e;
+  uint32_t* hl = s->B.health;
   if (s->prec == 4) {
     e = dtype_bytes == 4
-            ? sim_set_state<float, float>((const float*)dp, (const float*)dv, cur_state<float>(s), s->L, s->stream)
-            : sim_set_state<float, double>((const double*)dp, (const double*)dv, cur_state<float>(s), s->L, s->stream);
+            ? sim_set_state<float, float>((const float*)dp, (const float*)dv, cur_state<float>(s), s->L, s->stream, hl)
+            : sim_set_state<float, double>((const double*)dp, (const double*)dv, cur_state<float>(s), s->L, s->stream,
+                                           hl);
   } else {
     e = dtype_bytes == 4
-            ? sim_set_state<double, float>((const float*)dp, (const float*)dv, cur_state<double>(s), s->L, s->stream)
+            ? sim_set_state<double, float>((const float*)dp, (const float*)dv, cur_state<double>(s), s->L, s->stream,
+                                           hl)
             : sim_set_state<double, double>((const double*)dp, (const double*)dv, cur_state<double>(s), s->L,
-                                            s->stream);
+                                            s->stream, hl);
   }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->h_health, s->B.health, sizeof(uint32_t) * PM_HEALTH_COUNT, hipMemcpyDeviceToHost, s->stream);
   hipError_t es = hipStreamSynchronize(s->stream);
   if (tmp) (void)hipFree(tmp);
   if (e != hipSuccess || es != hipSuccess) return fail(PM_E_HIP, "set_state failed: %s", hipGetErrorString(e ? e : es));
   s->has_state = true;
   s->bins_valid = false;
-  s->rho_valid = false;
+  s->rho_valid = s->spec_valid = false;
   s->phi_valid = false;
+  if (int rc = health_verdict(s, s->stream)) {
+    s->has_state = false;  // a state with non-finite values is not kept
+    return rc;
+  }
   return PM_OK;
 }
 
@@ -1015,13 +1118,13 @@ PM_API int pm_sim_get_density(pm_sim* s, void* rho, int dtype_bytes, int on_devi
     int rc = s->prec == 4 ? sim_deposit_impl<float>(s) : sim_deposit_impl<double>(s);
     if (rc) return rc;
   }
-  return copy_grid_out(s, s->rho, rho, dtype_bytes, on_device);
+  return copy_grid_out(s, rho_slot(s), rho, dtype_bytes, on_device);
 }
 
 PM_API int pm_sim_get_potential(pm_sim* s, void* phi, int dtype_bytes, int on_device) {
   if (!s || !phi) return fail(PM_E_INVALID, "null argument");
   if (!s->phi_valid) return fail(PM_E_STATE, "no potential computed yet (run a step first)");
-  return copy_grid_out(s, s->H, phi, dtype_bytes, on_device);
+  return copy_grid_out(s, spec_slot(s), phi, dtype_bytes, on_device);
 }
 
 PM_API int pm_sim_step(pm_sim* s, double a, double da) {
@@ -1030,8 +1133,9 @@ PM_API int pm_sim_step(pm_sim* s, double a, double da) {
   if (!(a > 0.0) || !(a + da > 0.0)) return fail(PM_E_INVALID, "scale factor must be positive");
   int rc = s->prec == 4 ? sim_step_impl<float>(s, a, da) : sim_step_impl<double>(s, a, da);
   if (rc) return rc;
+  if (int rc2 = health_fetch(s, s->stream)) return rc2;
   PM_HIP(hipStreamSynchronize(s->stream));
-  return PM_OK;
+  return health_verdict(s, s->stream);
 }
 
 PM_API int pm_sim_run(pm_sim* s, double a, double da, int n_steps, double* a_out) {
@@ -1050,7 +1154,9 @@ PM_API int pm_sim_run(pm_sim* s, double a, double da, int n_steps, double* a_out
     if (rc) return rc;
     a += da;  // pmesh.py:63
   }
+  if (int rc = health_fetch(s, s->stream)) return rc;
   PM_HIP(hipStreamSynchronize(s->stream));
+  if (int rc = health_verdict(s, s->stream)) return rc;
   if (a_out) *a_out = a;
   return PM_OK;
 }
@@ -1110,11 +1216,11 @@ PM_API int pm_sim_zeldovich(pm_sim* s, const double* amp, int64_t n_amp, double 
   if (e == hipSuccess) {
     if (s->prec == 4)
       e = zeldovich_ics<float>(noise, noise_bytes, seed, d_amp, psi_scale, vel_factor, s->L, *t,
-                               (float*)s->sb[s->cur], (float*)s->vb[s->cur], (float*)s->rho, (float*)s->H,
+                               (float*)s->sb[s->cur], (float*)s->vb[s->cur], (float*)rho_slot(s), (float*)spec_slot(s),
                                (cpx<float>*)s->Q, s->stream);
     else
       e = zeldovich_ics<double>(noise, noise_bytes, seed, d_amp, psi_scale, vel_factor, s->L, *t,
-                                (double*)s->sb[s->cur], (double*)s->vb[s->cur], (double*)s->rho, (double*)s->H,
+                                (double*)s->sb[s->cur], (double*)s->vb[s->cur], (double*)rho_slot(s), (double*)spec_slot(s),
                                 (cpx<double>*)s->Q, s->stream);
     if (e == hipSuccess) e = ids_identity(s->idb[s->cur], s->L, s->stream);  // the state is in slot order
   }
@@ -1123,7 +1229,7 @@ PM_API int pm_sim_zeldovich(pm_sim* s, const double* amp, int64_t n_amp, double 
   if (e != hipSuccess || es != hipSuccess)
     return fail(PM_E_HIP, "device initial conditions failed: %s", hipGetErrorString(e ? e : es));
   s->has_state = true;
-  s->rho_valid = s->bins_valid = s->phi_valid = false;
+  s->rho_valid = s->bins_valid = s->phi_valid = s->spec_valid = false;
   return PM_OK;
 }
 
@@ -1145,18 +1251,18 @@ PM_API int pm_sim_power_spectrum(pm_sim* s, double box, int n_bins, double* k, d
   hipError_t e = hipMemsetAsync(d, 0, sizeof(double) * nd, s->stream);
   if (e == hipSuccess) {
     if (s->prec == 4) {
-      cpx<float>* H = (cpx<float>*)s->H;
+      cpx<float>* H = (cpx<float>*)spec_slot(s);
       cpx<float>* Q = (cpx<float>*)s->Q;
-      e = grid_sum<float>((const float*)s->rho, cells, d, s->stream);
-      if (!e) e = fft_r2c_x<float, float>((const float*)s->rho, H, Q, n, *s->tab, s->stream);
+      e = grid_sum<float>((const float*)rho_slot(s), cells, d, s->stream);
+      if (!e) e = fft_r2c_x<float, float>((const float*)rho_slot(s), H, Q, n, *s->tab, s->stream);
       if (!e) e = fft_c2c_y<float>(H, Q, n, -1, *s->tab, s->stream);
       if (!e) e = fft_c2c_z<float>(H, Q, n, -1, *s->tab, s->stream);
       if (!e) e = shell_sums<float>(H, Q, n, n_bins, d + 1, s->stream);
     } else {
-      cpx<double>* H = (cpx<double>*)s->H;
+      cpx<double>* H = (cpx<double>*)spec_slot(s);
       cpx<double>* Q = (cpx<double>*)s->Q;
-      e = grid_sum<double>((const double*)s->rho, cells, d, s->stream);
-      if (!e) e = fft_r2c_x<double, double>((const double*)s->rho, H, Q, n, *s->tab, s->stream);
+      e = grid_sum<double>((const double*)rho_slot(s), cells, d, s->stream);
+      if (!e) e = fft_r2c_x<double, double>((const double*)rho_slot(s), H, Q, n, *s->tab, s->stream);
       if (!e) e = fft_c2c_y<double>(H, Q, n, -1, *s->tab, s->stream);
       if (!e) e = fft_c2c_z<double>(H, Q, n, -1, *s->tab, s->stream);
       if (!e) e = shell_sums<double>(H, Q, n, n_bins, d + 1, s->stream);
@@ -1247,8 +1353,8 @@ void ens_describe(pm_ensemble* e) {
     const pm_sim* s = e->members[e->order[i]];
     BatchMember& m = D.m[i];
     const int n = s->L.n;
-    m.rho = (float*)s->rho;
-    m.H = (cpx<float>*)s->H;
+    m.rho = (float*)rho_slot(s);
+    m.H = (cpx<float>*)spec_slot(s);
     m.Q = (cpx<float>*)s->Q;
     m.tw = (const cpx<float>*)s->tab->tw;
     m.sin2 = (const float*)s->tab->sin2;
@@ -1275,6 +1381,8 @@ void ens_describe(pm_ensemble* e) {
     m.entries = s->B.entries;
     m.block_sums = s->B.block_sums;
     m.scan_done = s->B.done;
+    m.health = s->B.health;
+    m.det_fix = det_fix(s);
     m.scan_nb = scan_blocks(s->G);
     D.max_n = std::max(D.max_n, n);
     long blocks[BS_COUNT];
@@ -1513,15 +1621,27 @@ PM_API int pm_ensemble_run(pm_ensemble* e, double* a, const double* da, int n_st
     for (; k < chunk; ++k)
       if (int rc = ens_step(e)) return rc;
     // the host table must outlive the asynchronous copies above
+    for (pm_sim* s : e->members)
+      if (int rc = health_fetch(s, e->stream)) return rc;
     PM_HIP(hipStreamSynchronize(e->stream));
     done += chunk;
+    bool sick = false;
+    for (pm_sim* s : e->members)
+      sick = sick || (s->h_health[PM_HEALTH_NONFINITE] | s->h_health[PM_HEALTH_BINS] | s->h_health[PM_HEALTH_OVERFLOW]);
+    if (sick) break;  // the steps done so far are kept; the member is named below
   }
   for (int i = 0; i < nm; ++i) {
     a[i] = acur[i];
     pm_sim* s = e->members[i];
-    s->cur ^= n_steps & 1;  // every step's gather switched the member's state buffer
+    s->cur ^= done & 1;  // every step's gather switched the member's state buffer
     s->rho_valid = s->bins_valid = s->phi_valid = true;
+    s->spec_valid = false;
   }
+  for (int i = 0; i < nm; ++i)
+    if (int rc = health_verdict(e->members[i], e->stream)) {
+      g_err = "ensemble member " + std::to_string(i) + " (after " + std::to_string(done) + " steps): " + g_err;
+      return rc;
+    }
   return PM_OK;
 }
 

@@ -47,6 +47,8 @@ struct BatchMember {
   uint32_t* entries;
   uint32_t* block_sums;
   uint32_t* scan_done;
+  uint32_t* health;  // the member's PM_HEALTH_* counters
+  double det_fix;    // > 0: deterministic fixed-point deposit with this many units per unit mass
   int scan_nb;  // scan chunks of this member
 };
 

@@ -101,3 +101,11 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
 #define PM_E_NOMEM 3        // device allocation failed
 #define PM_E_STATE 4        // call out of order (e.g. step before state is set)
 #define PM_E_UNSUPPORTED 5  // feature not available for this configuration
+#define PM_E_CORRUPT 6      // the step found non-finite particles or inconsistent tile bins
+
+// Health counters a sim keeps in device memory; kernels add to them only when they find a
+// problem, and the host checks them at the synchronisation that ends a step or a run.
+#define PM_HEALTH_NONFINITE 0  // particles whose new position or velocity is not finite
+#define PM_HEALTH_BINS 1       // own entries whose cell lies outside their tile (stale bins)
+#define PM_HEALTH_OVERFLOW 2   // entries that found no place in their bin (count mismatch)
+#define PM_HEALTH_COUNT 4

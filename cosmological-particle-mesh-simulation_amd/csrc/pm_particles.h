@@ -74,7 +74,7 @@ __host__ __device__ __forceinline__ void slot_to_ijk(long slot, const Lattice& L
 }
 
 // Stage timers recorded by the sim (indices into the timing arrays).
-enum Stage { ST_DEPOSIT = 0, ST_R2C, ST_Y_FWD, ST_Z_GREEN, ST_Y_INV, ST_C2R, ST_GKD, ST_BIN, ST_COUNT };
+enum Stage { ST_DEPOSIT = 0, ST_R2C, ST_Y_FWD, ST_Z_GREEN, ST_Y_INV, ST_C2R, ST_GKD, ST_BIN, ST_DEPOSIT_R2C, ST_COUNT };
 
 // Eulerian tiles of the grid (cells per tile and tiles per axis).
 struct TileGeom {
@@ -94,6 +94,7 @@ struct Bins {
   uint32_t* block_sums = nullptr;  // scan spine
   uint32_t* done = nullptr;        // ticket counter of the scan's last-block spine (zero between scans)
   uint32_t capacity = 0;           // entries allocated (8*N_p)
+  uint32_t* health = nullptr;      // PM_HEALTH_COUNT counters of problems the kernels found
 };
 
 // Tile shapes (x, y, z cells) the tile kernels are instantiated for; the autotuner picks
@@ -116,8 +117,10 @@ template <typename R> struct PState {
 
 // ---- resident-state kernels (displacement storage) -------------------------------------------
 // set: stores slot order (id = storage index); get: writes every particle at its reference index
+// set_state counts non-finite inputs into health[PM_HEALTH_NONFINITE] (health may be null)
 template <typename R, typename Tin>
-hipError_t sim_set_state(const Tin* pos, const Tin* vel, const PState<R>& p, const Lattice& L, hipStream_t st);
+hipError_t sim_set_state(const Tin* pos, const Tin* vel, const PState<R>& p, const Lattice& L, hipStream_t st,
+                         uint32_t* health = nullptr);
 template <typename R, typename Tout>
 hipError_t sim_get_state(const PState<R>& p, Tout* pos, Tout* vel, const Lattice& L, hipStream_t st);
 // id = storage index (slot order)
@@ -129,10 +132,21 @@ hipError_t bins_count(const PState<R>& p, const Lattice& L, const TileGeom& G, c
 hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st);
 template <typename R>
 hipError_t bins_scatter(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
-// step (optional): a table-driven step's counter, advanced by the deposit as the step's last kernel
+// step (optional): a table-driven step's counter, advanced by the deposit as the step's last kernel.
+// det_fix > 0: order-independent fixed-point sums with det_fix units per unit mass
+// (deterministic_fix gives the one for a total mass).
 template <typename R>
 hipError_t deposit_tiles(const PState<R>& p, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
-                         hipStream_t st, int* step = nullptr);
+                         hipStream_t st, int* step = nullptr, double det_fix = 0.0);
+// the largest power of two f with (total mass + one unit per particle corner) * f < 2^62
+double deterministic_fix(double total_mass, long n_particles);
+// Fused deposit + R2C x pass (FP32): the density of the binned particles, transformed along x,
+// written as the half spectrum H (N^2 rows of N/2 complex) and the Nyquist plane Q; the density
+// grid itself is never stored.  Needs tiles of 8 x 8 cells in y and z; supported() tells whether
+// the grid size and tile shape have an instantiation.
+bool deposit_r2c_supported(int n, const TileGeom& G);
+hipError_t deposit_r2c(const PState<float>& p, float mass, const Lattice& L, const TileGeom& G, const Bins& B,
+                       const FftTables& t, cpx<float>* H, cpx<float>* Q, hipStream_t st, int* step = nullptr);
 // Reads `in` through the bins' own entries and writes the updated particles to `out` at their
 // own-entry index (never in place); also counts the entries of the new positions into B.counts.
 template <typename R>

@@ -20,7 +20,7 @@ inline unsigned grid_for(long n, int block = kBlock) {
 template <typename R, typename Tin>
 __global__ __launch_bounds__(kBlock) void k_set_state(const Tin* __restrict__ pos, const Tin* __restrict__ vel,
                                                       R* __restrict__ s, R* __restrict__ v, uint32_t* __restrict__ id,
-                                                      Lattice L) {
+                                                      Lattice L, uint32_t* __restrict__ health) {
   const long slot = blockIdx.x * (long)kBlock + threadIdx.x;
   if (slot >= L.np) return;
   id[slot] = (uint32_t)slot;
@@ -28,15 +28,19 @@ __global__ __launch_bounds__(kBlock) void k_set_state(const Tin* __restrict__ po
   slot_to_ijk(slot, L, ijk[0], ijk[1], ijk[2]);
   const long p = ((long)ijk[0] * L.np_side + ijk[1]) * L.np_side + ijk[2];
   const double n = L.n, h = 0.5 * L.n;
+  bool finite = true;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double x = (double)pos[d * L.np + p];
+    const double vd = (double)vel[d * L.np + p];
+    finite = finite && __builtin_isfinite(x) && __builtin_isfinite(vd);
     double sd = x - ((double)ijk[d] * L.r + 0.5);
     if (sd >= h) sd -= n;
     else if (sd < -h) sd += n;
     s[d * L.np + slot] = R(sd);
-    v[d * L.np + slot] = R(vel[d * L.np + p]);
+    v[d * L.np + slot] = R(vd);
   }
+  if (!finite && health) atomicAdd(&health[PM_HEALTH_NONFINITE], 1u);
 }
 
 // storage index e holds particle id[e]; ids outside the lattice are skipped
@@ -284,8 +288,9 @@ __global__ __launch_bounds__(kBlock) void k_convert(const Tin* __restrict__ in, 
 // host launchers
 // ------------------------------------------------------------------------------------------
 template <typename R, typename Tin>
-hipError_t sim_set_state(const Tin* pos, const Tin* vel, const PState<R>& p, const Lattice& L, hipStream_t st) {
-  k_set_state<R, Tin><<<grid_for(L.np), kBlock, 0, st>>>(pos, vel, p.s, p.v, p.id, L);
+hipError_t sim_set_state(const Tin* pos, const Tin* vel, const PState<R>& p, const Lattice& L, hipStream_t st,
+                         uint32_t* health) {
+  k_set_state<R, Tin><<<grid_for(L.np), kBlock, 0, st>>>(pos, vel, p.s, p.v, p.id, L, health);
   return hipGetLastError();
 }
 template <typename R, typename Tout>
@@ -298,7 +303,8 @@ hipError_t ids_identity(uint32_t* id, const Lattice& L, hipStream_t st) {
   return hipGetLastError();
 }
 #define PM_INST(R, T)                                                                                  \
-  template hipError_t sim_set_state<R, T>(const T*, const T*, const PState<R>&, const Lattice&, hipStream_t); \
+  template hipError_t sim_set_state<R, T>(const T*, const T*, const PState<R>&, const Lattice&, hipStream_t,   \
+                                          uint32_t*);                                                          \
   template hipError_t sim_get_state<R, T>(const PState<R>&, T*, T*, const Lattice&, hipStream_t);
 PM_INST(float, float)
 PM_INST(float, double)

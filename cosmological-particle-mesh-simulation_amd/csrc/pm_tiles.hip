@@ -19,6 +19,7 @@
 // staged in LDS.  Counts come from a wave-aggregated atomic per distinct key (one atomic per
 // distinct tile in a wavefront), and the offsets from a three-kernel exclusive scan.
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "pm_batch.h"
@@ -204,15 +205,23 @@ template <typename R, int TX, int TY, int TZ>
 __device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const uint32_t* __restrict__ id,
                                                  const Lattice& L, const TileGeom& G,
                                                  const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursors,
-                                                 uint32_t* __restrict__ entries, uint32_t capacity, long blk) {
+                                                 uint32_t* __restrict__ entries, uint32_t capacity, long blk,
+                                                 uint32_t* __restrict__ health) {
   const long slot = blk * (long)kT + threadIdx.x;  // storage index
   const bool valid = slot < L.np;
   int c[3] = {0, 0, 0}, t[3] = {0, 0, 0}, up[3] = {0, 0, 0};
   R d[3];
+  bool finite = true;
   if (valid) {
-    cell_and_frac<R>(s, slot, id[slot], L, c, d);
+    const R x0 = s[slot], x1 = s[L.np + slot], x2 = s[2 * L.np + slot];
+    finite = __builtin_isfinite(x0) && __builtin_isfinite(x1) && __builtin_isfinite(x2);
+    R q[3];
+    cell_and_frac<R>(x0, x1, x2, id[slot], L, c, d, q);
     tile_of<TX, TY, TZ>(c, G, t, up);
   }
+  // a non-finite displacement (from a non-finite velocity in the drift, too) is binned at the
+  // clamped cell and counted
+  if (!finite && health) atomicAdd(&health[PM_HEALTH_NONFINITE], 1u);
   // halo keys only in wavefronts that have any (wave-uniform)
   const bool any_halo = __ballot(valid && (up[0] | up[1] | up[2])) != 0ull;
   uint32_t key[8], beg[8], lim[8];
@@ -234,25 +243,31 @@ __device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const 
 #pragma unroll
     for (int o = 1; o < 8; ++o) halo[o - 1].issue(cursors, key[o], act[o]);
   }
+  uint32_t lost = 0;  // entries beyond their segment: the counts disagree with the positions
   if (act[0]) {
     const uint32_t pos = beg[0] + home.finish();
     if (pos < lim[0] && pos < capacity) entries[pos] = (uint32_t)slot;
+    else ++lost;
   }
   if (any_halo) {
 #pragma unroll
     for (int o = 1; o < 8; ++o) {
       const uint32_t pos = beg[o] + halo[o - 1].finish();
-      if (act[o] && pos < lim[o] && pos < capacity) entries[pos] = (uint32_t)slot;
+      if (act[o]) {
+        if (pos < lim[o] && pos < capacity) entries[pos] = (uint32_t)slot;
+        else ++lost;
+      }
     }
   }
+  if (lost && health) atomicAdd(&health[PM_HEALTH_OVERFLOW], lost);
 }
 
 template <typename R, int TX, int TY, int TZ>
 __global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, const uint32_t* __restrict__ id,
                                                     Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
                                                     uint32_t* __restrict__ cursors, uint32_t* __restrict__ entries,
-                                                    uint32_t capacity) {
-  bin_scatter_body<R, TX, TY, TZ>(s, id, L, G, offsets, cursors, entries, capacity, blockIdx.x);
+                                                    uint32_t capacity, uint32_t* __restrict__ health) {
+  bin_scatter_body<R, TX, TY, TZ>(s, id, L, G, offsets, cursors, entries, capacity, blockIdx.x, health);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -381,12 +396,16 @@ __global__ __launch_bounds__(kT) void k_scan_apply(uint32_t* __restrict__ counts
 // banks instead of sharing the few that a power-of-two pitch leaves them
 template <int TX, int TY, int TZ> constexpr int deposit_lds_cells() { return (TX + 3) * TY * TZ; }
 
-template <typename R, int TX, int TY, int TZ>
+// DET: the LDS sums are 64-bit fixed point with `fix` units per unit mass (a power of two chosen
+// by the host so that the whole mass fits in 62 bits); integer adds are associative, so the
+// cell values do not depend on the order the atomics land in.
+template <typename R, int TX, int TY, int TZ, bool DET = false>
 __device__ __forceinline__ void deposit_body(const R* __restrict__ s, const uint32_t* __restrict__ id,
                                              R* __restrict__ rho, R mass, const Lattice& L, const TileGeom& G,
                                              const uint32_t* __restrict__ offsets,
                                              const uint32_t* __restrict__ entries, uint32_t capacity, int tile,
-                                             double* acc) {
+                                             double* acc, uint32_t* __restrict__ health, double fix = 0.0) {
+  unsigned long long* iacc = reinterpret_cast<unsigned long long*>(acc);
   constexpr int CELLS = TX * TY * TZ;
   static_assert(CELLS % kT == 0, "tile must hold a multiple of the block size");
   constexpr int E = 4;
@@ -418,8 +437,12 @@ __device__ __forceinline__ void deposit_body(const R* __restrict__ s, const uint
     }
   };
   load(0);  // the first batch is in flight while the accumulator is cleared
-  for (int i = threadIdx.x; i < PADDED; i += kT) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < PADDED; i += kT) {
+    if (DET) iacc[i] = 0ull;
+    else acc[i] = 0.0;
+  }
   __syncthreads();
+  uint32_t bad = 0;  // own entries whose cell lies outside the tile (stale bins)
   for (uint32_t base = 0; base < total; base += kT * E) {
     if (base != 0) load(base);
 #pragma unroll
@@ -428,6 +451,8 @@ __device__ __forceinline__ void deposit_body(const R* __restrict__ s, const uint
       int c[3];
       R d[3], q[3];
       cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], pid[u], L, c, d, q);
+      if (base + u * kT + threadIdx.x < n_own)
+        bad += (c[0] / TX != tix || c[1] / TY != tiy || c[2] / TZ != tiz) ? 1u : 0u;
       int r0[3], r1[3];  // tile-relative coordinates of the two corners per axis (-1: outside)
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -455,34 +480,210 @@ __device__ __forceinline__ void deposit_body(const R* __restrict__ s, const uint
           for (int ox = 0; ox < 2; ++ox) {
             const int rx = ox ? r1[0] : r0[0];
             if (rx < 0) continue;
-            __hip_atomic_fetch_add(&acc[rz * PLANE + ry * PX + rx], wyz * (ox ? dd[0] : w0[0]), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            const double w = wyz * (ox ? dd[0] : w0[0]);
+            if (DET)
+              __hip_atomic_fetch_add(&iacc[rz * PLANE + ry * PX + rx], (unsigned long long)__double2ll_rn(w * fix),
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            else
+              __hip_atomic_fetch_add(&acc[rz * PLANE + ry * PX + rx], w, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         }
       }
     }
   }
+  if (bad && health) atomicAdd(&health[PM_HEALTH_BINS], bad);
   __syncthreads();
   const size_t nn = (size_t)n;
 #pragma unroll
   for (int i0 = 0; i0 < CELLS / kT; ++i0) {
     const int i = i0 * kT + threadIdx.x;
     const int rx = i % TX, ry = (i / TX) % TY, rz = i / (TX * TY);
-    rho[((size_t)(o0[2] + rz) * nn + (o0[1] + ry)) * nn + (o0[0] + rx)] = R(acc[rz * PLANE + ry * PX + rx]);
+    const int li = rz * PLANE + ry * PX + rx;
+    rho[((size_t)(o0[2] + rz) * nn + (o0[1] + ry)) * nn + (o0[0] + rx)] =
+        DET ? R((double)iacc[li] / fix) : R(acc[li]);
   }
 }
 
 // step: when set, the last kernel of a table-driven step advances the table's step counter
 // (no kernel of the step reads it after the gather)
-template <typename R, int TX, int TY, int TZ>
+template <typename R, int TX, int TY, int TZ, bool DET>
 __global__ __launch_bounds__(kT) void k_deposit_tiles(const R* __restrict__ s, const uint32_t* __restrict__ id,
                                                       R* __restrict__ rho, R mass, Lattice L, TileGeom G,
                                                       const uint32_t* __restrict__ offsets,
                                                       const uint32_t* __restrict__ entries, uint32_t capacity,
-                                                      int* step) {
+                                                      int* step, uint32_t* __restrict__ health, double fix) {
   __shared__ double acc[deposit_lds_cells<TX, TY, TZ>()];
-  deposit_body<R, TX, TY, TZ>(s, id, rho, mass, L, G, offsets, entries, capacity, xcd_tile(blockIdx.x, gridDim.x),
-                              acc);
+  deposit_body<R, TX, TY, TZ, DET>(s, id, rho, mass, L, G, offsets, entries, capacity,
+                                   xcd_tile(blockIdx.x, gridDim.x), acc, health, fix);
+  if (step && blockIdx.x == 0 && threadIdx.x == 0) *step += 1;
+}
+
+// ------------------------------------------------------------------------------------------
+// fused deposit + real-to-complex x transform (density.py:17-47, then potential.py:19-21 along
+// x), FP32
+//
+// One workgroup owns a row block: the NTX tiles of one (tile y, tile z) pair, i.e. TY * TZ full
+// grid rows of N cells.  It deposits the own and halo entries of those tiles (each entry's
+// corners that fall in the tile whose list holds it, as the tile deposit does) into an LDS image
+// of the rows, then transforms the rows along x exactly as the R2C pass does (N/2-point complex
+// FFT of x[2n] + i x[2n+1] and the even/odd untangling) and writes the half spectrum and the
+// Nyquist plane.  The density grid is never written: the pass saves one write and one read of
+// N^3 reals per step.  The LDS image is the transform's own layout (complex n of row w at
+// n * (RB + 1) + w), so the accumulation needs no reshuffle; it is FP32 (the FP64 accumulator
+// of the tile deposit would not fit the rows' LDS).
+// ------------------------------------------------------------------------------------------
+constexpr int kDrT = 1024;  // threads per row-block workgroup
+
+template <int N, int RB> constexpr int deposit_r2c_lds() { return ((N / 2) * (RB + 1) + N) * 8 + 4 * 80; }
+
+template <int TX, int TY, int TZ, int N>
+__global__ __launch_bounds__(kDrT) void k_deposit_r2c(const float* __restrict__ s, const uint32_t* __restrict__ id,
+                                                      float mass, Lattice L, TileGeom G,
+                                                      const uint32_t* __restrict__ offsets,
+                                                      const uint32_t* __restrict__ entries, uint32_t capacity,
+                                                      const float2* __restrict__ twg, float2* __restrict__ H,
+                                                      float2* __restrict__ Q, int* step, uint32_t* __restrict__ health) {
+  constexpr int M = N / 2, RB = TY * TZ, SN = RB + 1, NTX = N / TX, NT = kDrT;
+  constexpr int E = 2;  // entries per thread and batch (all loads issued before use)
+  static_assert(NTX <= 64, "row block of at most 64 tiles");
+  __shared__ float2 lds[M * SN];
+  __shared__ float2 tw[N];
+  __shared__ uint32_t hoff[NTX + 1];
+  float* acc = reinterpret_cast<float*>(lds);
+  const int tid = threadIdx.x;
+  const int rb = xcd_block(blockIdx.x, gridDim.x);  // = tile z * nty + tile y
+  const int ty = rb % G.nty, tz = rb / G.nty;
+  const int y0 = ty * TY, z0 = tz * TZ;
+  const uint32_t t0 = (uint32_t)rb * NTX, nt = (uint32_t)G.ntiles;
+  const uint32_t oe = min(offsets[t0 + NTX], capacity), ob = min(offsets[t0], oe);
+  const uint32_t he = min(offsets[nt + t0 + NTX], capacity), hb = min(offsets[nt + t0], he);
+  const uint32_t n_own = oe - ob, total = n_own + (he - hb);
+  if (tid <= NTX) hoff[tid] = offsets[nt + t0 + tid];
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  for (int i = tid; i < M * SN; i += NT) lds[i] = make_float2(0.f, 0.f);
+  const long np = L.np;
+  const int n = L.n;
+  uint32_t bad = 0;  // own entries whose cell lies outside the row block (stale bins)
+  __syncthreads();
+  for (uint32_t base = 0; base < total; base += NT * E) {
+    uint32_t sl[E], pid[E];
+    float sv[E][3];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const uint32_t i = base + u * NT + tid;
+      const uint32_t e = i < n_own ? ob + i : hb + (i - n_own);
+      sl[u] = i < total ? min(entries[e], (uint32_t)(np - 1)) : kNone;
+    }
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      pid[u] = sl[u] != kNone ? id[sl[u]] : 0u;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) sv[u][a] = sl[u] != kNone ? s[a * np + sl[u]] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      if (sl[u] == kNone) continue;
+      const uint32_t i = base + u * NT + tid;
+      int c[3];
+      float d[3], q[3];
+      cell_and_frac<float>(sv[u][0], sv[u][1], sv[u][2], pid[u], L, c, d, q);
+      // the tile whose list holds the entry: the home tile for own entries, the halo list's
+      // tile (largest j with hoff[j] <= entry) for halo entries
+      int tx;
+      if (i < n_own) {
+        tx = c[0] / TX;
+        bad += (c[1] / TY != ty || c[2] / TZ != tz) ? 1u : 0u;
+      } else {
+        const uint32_t e = hb + (i - n_own);
+        int lo = 0, hi = NTX - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (hoff[mid] <= e) lo = mid;
+          else hi = mid - 1;
+        }
+        tx = lo;
+      }
+      const int o0[3] = {tx * TX, y0, z0};
+      const int T[3] = {TX, TY, TZ};
+      int r0[3], r1[3];  // tile-relative coordinates of the two corners per axis (-1: outside)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        int lo = c[a] - o0[a];
+        if (lo < 0) lo += n;
+        int hi = (c[a] + 1 == n ? 0 : c[a] + 1) - o0[a];
+        if (hi < 0) hi += n;
+        r0[a] = lo < T[a] ? lo : -1;
+        r1[a] = hi < T[a] ? hi : -1;
+      }
+      // density.py:24-47: weights t = 1 - d and d times the mass
+      const float w0[3] = {1.f - d[0], 1.f - d[1], 1.f - d[2]};
+#pragma unroll
+      for (int oz = 0; oz < 2; ++oz) {
+        const int rz = oz ? r1[2] : r0[2];
+        if (rz < 0) continue;
+        const float wz = mass * (oz ? d[2] : w0[2]);
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) {
+          const int ry = oy ? r1[1] : r0[1];
+          if (ry < 0) continue;
+          const float wyz = wz * (oy ? d[1] : w0[1]);
+          const int w = rz * TY + ry;
+#pragma unroll
+          for (int ox = 0; ox < 2; ++ox) {
+            const int rx = ox ? r1[0] : r0[0];
+            if (rx < 0) continue;
+            const int x = o0[0] + rx;
+            __hip_atomic_fetch_add(&acc[2 * ((x >> 1) * SN + w) + (x & 1)], wyz * (ox ? d[0] : w0[0]),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        }
+      }
+    }
+  }
+  if (bad && health) atomicAdd(&health[PM_HEALTH_BINS], bad);
+  __syncthreads();
+  // rows along x: N/2-point complex FFT, then the even/odd untangling (as r2c_body)
+  fft_passes<float2, M, 1, -1, RB, SN, NT, 2>(lds, tw, tid);
+  constexpr int NPAIR = M / 2 + 1;
+  constexpr int TOTP = NPAIR * RB;
+  constexpr int PERP = (TOTP + NT - 1) / NT;
+  float2 zk[PERP], zm[PERP];
+#pragma unroll
+  for (int i = 0; i < PERP; ++i) {
+    const int b = tid + i * NT;
+    if (b < TOTP) {
+      const int w = b % RB, k = b / RB;
+      zk[i] = lds[k * SN + w];
+      zm[i] = lds[((M - k) % M) * SN + w];
+    }
+  }
+  __syncthreads();
+  const size_t nn = (size_t)N;
+#pragma unroll
+  for (int i = 0; i < PERP; ++i) {
+    const int b = tid + i * NT;
+    if (b < TOTP) {
+      const int w = b % RB, k = b / RB;
+      const float2 a = zk[i], c = zm[i];
+      const float2 Ev = make_float2(0.5f * (a.x + c.x), 0.5f * (a.y - c.y));
+      const float2 Ov = make_float2(0.5f * (a.y + c.y), -0.5f * (a.x - c.x));
+      const float2 wO = cmul(Ov, tw[k]);
+      if (k == 0) {
+        lds[w] = make_float2(Ev.x + wO.x, 0.f);
+        Q[(size_t)(z0 + w / TY) * nn + (y0 + w % TY)] = make_float2(Ev.x - wO.x, 0.f);
+      } else {
+        lds[k * SN + w] = cadd(Ev, wO);
+        if (2 * k != M) lds[(M - k) * SN + w] = cconj(csub(Ev, wO));
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int e = tid; e < RB * M; e += NT) {
+    const int w = e / M, k = e - w * M;
+    H[((size_t)(z0 + w / TY) * nn + (y0 + w % TY)) * M + k] = lds[k * SN + w];
+  }
   if (step && blockIdx.x == 0 && threadIdx.x == 0) *step += 1;
 }
 
@@ -630,8 +831,9 @@ __device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restr
         R d[3], q[3];
         cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], pid[u], L, c, d, q);
         // tile-relative cell (box coordinate of the cell is that + 1).  Own entries were binned
-        // from these same positions, so the cell lies in the tile; the clamp only keeps a
-        // corrupted state inside the box.
+        // from these same positions, so the cell lies in the tile: the deposit that read the
+        // same bins counted any entry outside its tile into the health counters, and the clamp
+        // only keeps the reads of such an entry inside the box.
         int b[3];
         const int T[3] = {TX, TY, TZ};
 #pragma unroll
@@ -799,8 +1001,13 @@ __global__ __launch_bounds__(kT) void k_batch_deposit(const BatchDesc* __restric
   const int k = batch_member(D, BS_TILES, b);
   const BatchMember& m = D->m[k];
   const int cur = batch_state(D, m);
-  deposit_body<float, kBT, kBT, kBT>(m.s[cur], m.id[cur], m.rho, m.mass, m.L, m.G, m.offsets, m.entries, m.capacity,
-                                     (int)(b - D->off[BS_TILES][k]), acc);
+  const int tile = (int)(b - D->off[BS_TILES][k]);
+  if (m.det_fix > 0.0)
+    deposit_body<float, kBT, kBT, kBT, true>(m.s[cur], m.id[cur], m.rho, m.mass, m.L, m.G, m.offsets, m.entries,
+                                             m.capacity, tile, acc, m.health, m.det_fix);
+  else
+    deposit_body<float, kBT, kBT, kBT, false>(m.s[cur], m.id[cur], m.rho, m.mass, m.L, m.G, m.offsets, m.entries,
+                                              m.capacity, tile, acc, m.health);
 }
 
 __global__ __launch_bounds__(kT) void k_batch_scatter(const BatchDesc* __restrict__ D) {
@@ -809,7 +1016,7 @@ __global__ __launch_bounds__(kT) void k_batch_scatter(const BatchDesc* __restric
   const BatchMember& m = D->m[k];
   const int cur = batch_state(D, m);
   bin_scatter_body<float, kBT, kBT, kBT>(m.s[cur], m.id[cur], m.L, m.G, m.offsets, m.cursors, m.entries, m.capacity,
-                                         b - D->off[BS_SCATTER][k]);
+                                         b - D->off[BS_SCATTER][k], m.health);
 }
 
 __global__ __launch_bounds__(kT) void k_batch_scan_reduce(const BatchDesc* __restrict__ D) {
@@ -846,8 +1053,11 @@ bool tile_shape_supported(int tx, int ty, int tz) {
 
 TileGeom make_tiles(int n, int prec_bytes, int tx, int ty, int tz) {
   TileGeom g{};
-  (void)prec_bytes;  // the same shapes in both precisions (FP64 LDS accumulators in the deposit)
-  if (tx == 0 && ty == 0 && tz == 0) {  // default: 16^3 (8^3 on the smallest grids)
+  if (tx == 0 && ty == 0 && tz == 0 && prec_bytes == 4 && (n == 256 || n == 384 || n == 512)) {
+    // FP32 default where the fused deposit + R2C kernel runs: 32 x 8 x 8
+    tx = 32;
+    ty = tz = 8;
+  } else if (tx == 0 && ty == 0 && tz == 0) {  // default: 16^3 (8^3 on the smallest grids)
     auto pick = [n](int pref) {
       int t = pref;
       while (t > 1 && n % t) t /= 2;
@@ -914,7 +1124,7 @@ template <typename R>
 hipError_t bins_scatter(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st) {
 #define PM_L(X, Y, Z)                                                                                          \
   k_bin_scatter<R, X, Y, Z><<<blocks_for(L.np), kT, 0, st>>>(p.s, p.id, L, G, B.offsets, B.cursors, B.entries, \
-                                                             B.capacity)
+                                                             B.capacity, B.health)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
   return hipGetLastError();
@@ -922,10 +1132,16 @@ hipError_t bins_scatter(const PState<R>& p, const Lattice& L, const TileGeom& G,
 
 template <typename R>
 hipError_t deposit_tiles(const PState<R>& p, R* rho, double mass, const Lattice& L, const TileGeom& G, const Bins& B,
-                         hipStream_t st, int* step) {
-#define PM_L(X, Y, Z)                                                                                          \
-  k_deposit_tiles<R, X, Y, Z><<<G.ntiles, kT, 0, st>>>(p.s, p.id, rho, R(mass), L, G, B.offsets, B.entries,    \
-                                                       B.capacity, step)
+                         hipStream_t st, int* step, double det_fix) {
+#define PM_L(X, Y, Z)                                                                                              \
+  do {                                                                                                             \
+    if (det_fix > 0.0)                                                                                             \
+      k_deposit_tiles<R, X, Y, Z, true><<<G.ntiles, kT, 0, st>>>(p.s, p.id, rho, R(mass), L, G, B.offsets,         \
+                                                                 B.entries, B.capacity, step, B.health, det_fix);  \
+    else                                                                                                           \
+      k_deposit_tiles<R, X, Y, Z, false><<<G.ntiles, kT, 0, st>>>(p.s, p.id, rho, R(mass), L, G, B.offsets,        \
+                                                                  B.entries, B.capacity, step, B.health, 0.0);     \
+  } while (0)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
   return hipGetLastError();
@@ -949,6 +1165,41 @@ hipError_t gkd_tiles(const PState<R>& in, const PState<R>& out, const R* phi, do
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
   return hipGetLastError();
+}
+
+// grid sizes and tile shapes (x extent; 8 x 8 in y and z) of the fused deposit + R2C kernel
+#define PM_DR2C_CASES(X) X(256, 32) X(256, 64) X(384, 32) X(384, 64) X(512, 32) X(512, 64)
+
+bool deposit_r2c_supported(int n, const TileGeom& G) {
+  if (G.ty != 8 || G.tz != 8) return false;
+#define PM_C(NN, TXX) \
+  if (n == NN && G.tx == TXX) return true;
+  PM_DR2C_CASES(PM_C)
+#undef PM_C
+  return false;
+}
+
+hipError_t deposit_r2c(const PState<float>& p, float mass, const Lattice& L, const TileGeom& G, const Bins& B,
+                       const FftTables& t, cpx<float>* H, cpx<float>* Q, hipStream_t st, int* step) {
+  if (!deposit_r2c_supported(L.n, G) || t.n != L.n || t.prec != 4) return hipErrorInvalidValue;
+  const unsigned blocks = (unsigned)(G.nty * G.ntz);
+#define PM_C(NN, TXX)                                                                                              \
+  if (L.n == NN && G.tx == TXX) {                                                                                  \
+    k_deposit_r2c<TXX, 8, 8, NN><<<blocks, kDrT, 0, st>>>(p.s, p.id, mass, L, G, B.offsets, B.entries, B.capacity, \
+                                                          (const float2*)t.tw, H, Q, step, B.health);              \
+    return hipGetLastError();                                                                                      \
+  }
+  PM_DR2C_CASES(PM_C)
+#undef PM_C
+  return hipErrorInvalidValue;
+}
+
+double deterministic_fix(double total_mass, long n_particles) {
+  // each of the <= 8 corner contributions of a particle rounds by at most half a unit
+  const double room = std::ldexp(1.0, 62) - 4.0 * (double)n_particles;
+  int k = (int)std::floor(std::log2(room / total_mass));
+  k = std::min(k, 60);
+  return std::ldexp(1.0, k);
 }
 
 long batch_scatter_blocks(long np) { return blocks_for(np); }
@@ -995,7 +1246,7 @@ hipError_t batch_scan(const BatchDesc* d, long total, int n_members, hipStream_t
   template hipError_t bins_scatter<R>(const PState<R>&, const Lattice&, const TileGeom&, const Bins&,       \
                                       hipStream_t);                                                          \
   template hipError_t deposit_tiles<R>(const PState<R>&, R*, double, const Lattice&, const TileGeom&,       \
-                                       const Bins&, hipStream_t, int*);                                      \
+                                       const Bins&, hipStream_t, int*, double);                              \
   template hipError_t gkd_tiles<R>(const PState<R>&, const PState<R>&, const R*, double, double,            \
                                    const Lattice&, const TileGeom&, const Bins&, const StepTable&, hipStream_t);
 PM_INST(float)

@@ -21,6 +21,12 @@ from pmgpu.device import DeviceArray
 from pmgpu.sim import SimDensity, SimPositions, SimVelocities
 
 
+def _density_read_after_step() -> bool:
+    """Whether the configured driver writes or draws a step's density after the step."""
+    c = configure_me
+    return bool((c.SAVE_DATA and c.SAVE_DENSITY) or c.PLOT_STEPS or c.PLOT_PROJECTIONS)
+
+
 def advance_time(density, positions, velocities, fgrid, a, da):
     if isinstance(positions, SimPositions):
         sim = positions.sim
@@ -29,6 +35,8 @@ def advance_time(density, positions, velocities, fgrid, a, da):
         if not (isinstance(density, SimDensity) and density.sim is sim
                 and density.version == sim.version):
             sim.set_density(np.asarray(density))
+        elif _density_read_after_step():
+            density.keep()   # the step overwrites the density the caller may still save or plot
         if isinstance(fgrid, GreenGrid):
             sim.set_green(None)
         else:

@@ -1,97 +1,118 @@
-"""Simulation driver — the reference's pmesh.py:18-93 step loop, unchanged in structure.
+"""Command-line driver: build (or reload) the particles, then advance them with the PM step.
 
-``python pmesh.py`` runs the simulation configured in configure_me.py.  With the default
-configure_me.RESIDENT the particles and grids stay on the GPU for the whole loop; the host
-only sees them when a snapshot is written.
+``python pmesh.py`` reads configure_me.py and runs one simulation.  What the driver guarantees
+(the behaviour of the reference's driver, pmesh.py:18-93):
+
+* a step is ``density(positions, mass)`` followed by ``advance_time(...)``, with the code-unit
+  particle mass (N_CELLS/N_PARTS)^3 that makes the mean density 1 (pmesh.py:28);
+* the scale factor grows by da = (A_END - A_INIT)/STEPS, accumulated in float64, while
+  a < A_END - da, so rounding decides whether STEPS - 1 or STEPS steps run (pmesh.py:56,63);
+* output j is due after the first step that brings a to A_INIT + j*(A_END - A_INIT)/N or past
+  it (N = N_SAVE_FILES for snapshots, N_PLOTS for plots); snapshot 0 holds the initial
+  conditions; the density written or drawn with output j is the one that step solved for;
+* a restart from snapshot n continues with the snapshot counter at n and the plot counter at
+  n*N_PLOTS/N_SAVE_FILES.
+
+With configure_me.RESIDENT (the default) the particles stay on the GPU for the whole run and
+reach the host only when a snapshot is written.
 """
-from time import time
+from __future__ import annotations
 
-import configure_me as _c
+import time
+
+import configure_me as cfg
 from density import density
-from integrate import advance_time
-from zeldovich import zeldovich, new_simulation
 from fourier_utils import fourier_grid
 from gaussian_random_field import gaussian_random_field
-from cosmology import *  # noqa: F401,F403  (the reference imports the same names)
-from save_data import save_file, from_file
-from plot_helper import plot_step, plot_grf, plot_projection
+from integrate import advance_time
+from plot_helper import plot_grf, plot_projection, plot_step
+from save_data import from_file, save_file
+from zeldovich import new_simulation, zeldovich
+
+# h^2 M_sun / Mpc^3: critical density today, for the informational particle-mass line
+RHO_CRIT_H2 = 2.775e11
+
+
+class Schedule:
+    """Outputs spaced evenly in scale factor: ``due(a)`` is true once for every mark
+    a_first + j*spacing that a has reached, counting j from ``start``."""
+
+    def __init__(self, a_first: float, a_last: float, count: int, start: float = 0):
+        self.a_first = a_first
+        self.spacing = (a_last - a_first) / count
+        self.index = start
+
+    def due(self, a: float) -> bool:
+        if a >= self.a_first + self.index * self.spacing:
+            self.index += 1
+            return True
+        return False
+
+
+def _resident() -> bool:
+    return bool(getattr(cfg, "RESIDENT", True))
+
+
+def starting_point():
+    """(positions, velocities, a, first snapshot number, first plot number)."""
+    if cfg.RESTART:
+        n = cfg.RESTART_FROM_N
+        print(f"Resuming from snapshot {n}")
+        pos, vel, a = from_file(n)
+        if _resident():
+            pos, vel = new_simulation(pos, vel)
+        return pos, vel, float(a), n, n / cfg.N_SAVE_FILES * cfg.N_PLOTS
+    field = gaussian_random_field()
+    pos, vel = zeldovich(field)
+    first = 0
+    if cfg.SAVE_DATA:
+        save_file(field, pos, vel, 0, cfg.A_INIT)
+        first = 1
+    if cfg.PLOT_GRF:
+        plot_grf(field)
+    return pos, vel, cfg.A_INIT, first, 0
+
+
+def describe_run() -> None:
+    n_p = cfg.N_PARTS ** 3
+    m_sun = RHO_CRIT_H2 * cfg.OMEGA_M0 * cfg.H0 ** 2 * cfg.BOX_SIZE ** 3 / n_p
+    print(f"PM run: {cfg.N_PARTS}^3 particles, {cfg.N_CELLS}^3 cells, box {cfg.BOX_SIZE}, "
+          f"a {cfg.A_INIT} -> {cfg.A_END} in {cfg.STEPS} steps")
+    print(f"  particle mass {m_sun:.3E} M_sun, {(cfg.N_CELLS / cfg.N_PARTS) ** 3:.3e} in code units")
 
 
 def simulator():
-    N_CELLS, N_PARTS, BOX_SIZE = _c.N_CELLS, _c.N_PARTS, _c.BOX_SIZE
-    OMEGA_M0, H0, A_INIT, A_END = _c.OMEGA_M0, _c.H0, _c.A_INIT, _c.A_END
-    print('Starting the simulation for {}^3 particles'.format(N_PARTS), 'with {}^3 grid cells'.format(N_CELLS))
-
-    #Physical particle mass in solar masses
-    particle_mass = 1.32*10**5*(OMEGA_M0*H0**2)*(BOX_SIZE/(N_PARTS/128))**3
-    print('Particle mass in solar mass for this configuration: {:.3E}'.format(particle_mass))
-
-    dens_contrast = (N_CELLS/N_PARTS)**3
-    print('Particle mass in code units: {:.3e}'.format(dens_contrast))
-    da = (A_END - A_INIT)/_c.STEPS
-    da_save = (A_END - A_INIT)/_c.N_SAVE_FILES
-    da_plot = (A_END - A_INIT)/_c.N_PLOTS
-    a_current = A_INIT
-    n_file = 0
-    n_plot = 0
-
-    if _c.RESTART:
-        print("Restarting from file data.{}".format(_c.RESTART_FROM_N))
-        n_file = _c.RESTART_FROM_N
-        n_plot = (_c.RESTART_FROM_N)/_c.N_SAVE_FILES*_c.N_PLOTS
-        positions, velocities, a_current = from_file(n_file)
-        a_current = float(a_current)
-        if getattr(_c, "RESIDENT", True):
-            positions, velocities = new_simulation(positions, velocities)
-    else:
-        rho = gaussian_random_field()
-        positions, velocities = zeldovich(rho)
-        if _c.SAVE_DATA:
-            save_file(rho, positions, velocities, 0, a_current)
-            n_file += 1
-        if _c.PLOT_GRF:
-            plot_grf(rho)
-
-    ksq_inverse = fourier_grid()
-    print('Starting the integrations...')
+    """Runs the configured simulation; returns (positions, velocities, final a, steps taken)."""
+    describe_run()
+    mass = (cfg.N_CELLS / cfg.N_PARTS) ** 3
+    da = (cfg.A_END - cfg.A_INIT) / cfg.STEPS
+    positions, velocities, a, first_file, first_plot = starting_point()
+    snapshots = Schedule(cfg.A_INIT, cfg.A_END, cfg.N_SAVE_FILES, first_file)
+    plots = Schedule(cfg.A_INIT, cfg.A_END, cfg.N_PLOTS, first_plot)
+    green = fourier_grid()
+    span = cfg.A_END - cfg.A_INIT
     steps = 0
-    while a_current<A_END-da:
-
-        start_time = time()
-
-        rho = density(positions, dens_contrast)
-        positions, velocities = advance_time(rho, positions, velocities, ksq_inverse, a_current, da)
-
-        a_current += da
+    print("Stepping...")
+    while a < cfg.A_END - da:
+        t0 = time.perf_counter()
+        rho = density(positions, mass)
+        positions, velocities = advance_time(rho, positions, velocities, green, a, da)
+        a += da
         steps += 1
-
-        if a_current >= A_INIT + n_file*da_save:
-            if _c.SAVE_DATA:
-                save_file(rho, positions, velocities, n_file, a_current)
-            n_file += 1
-        if a_current >= A_INIT + n_plot*da_plot:
-            if _c.PLOT_STEPS:
-                plot_step(rho, n_plot)
-            if _c.PLOT_PROJECTIONS:
-                plot_projection(rho, n_plot, 15)
-            n_plot += 1
-
-        if _c.PRINT_STATUS:
-            print_status(a_current, start_time)
-
-    return positions, velocities, a_current, steps
-
-
-def print_status(a_current, start_time):
-
-    percentile = 100*(a_current-_c.A_INIT)/(_c.A_END-_c.A_INIT)
-    print("%.5f" % percentile,"%", " Save step time: --- %.5f seconds ---" % (time() - start_time))
+        if snapshots.due(a) and cfg.SAVE_DATA:
+            save_file(rho, positions, velocities, snapshots.index - 1, a)
+        if plots.due(a):
+            if cfg.PLOT_STEPS:
+                plot_step(rho, plots.index - 1)
+            if cfg.PLOT_PROJECTIONS:
+                plot_projection(rho, plots.index - 1, 15)
+        if cfg.PRINT_STATUS:
+            print(f"step {steps:5d}  a = {a:.5f}  ({100 * (a - cfg.A_INIT) / span:6.2f} %)  "
+                  f"{1e3 * (time.perf_counter() - t0):9.3f} ms")
+    return positions, velocities, a, steps
 
 
 if __name__ == "__main__":
-
-    start_time = time()
-
+    t_start = time.perf_counter()
     simulator()
-    print("Finished in")
-    print("--- %.2f seconds ---" % (time() - start_time))
+    print(f"Done in {time.perf_counter() - t_start:.2f} s")

@@ -16,10 +16,13 @@ PM_OK = 0
 PM_FLAG_CIC_DECONVOLVE = 1
 PM_FLAG_FIX_F_COSMOLOGY = 2
 PM_FLAG_NO_GRAPH = 4
+PM_FLAG_SEPARATE_DEPOSIT = 8
+PM_FLAG_DETERMINISTIC = 16
+PM_E_CORRUPT = 6
 
 # N stages timed by pm_sim_get_timing, in order
 STAGES = ("deposit", "r2c_x", "c2c_y_fwd", "z_green", "c2c_y_inv", "c2r_x", "gather_kick_drift",
-          "bin")
+          "bin", "deposit_r2c")
 
 _vp = C.c_void_p
 _i = C.c_int

@@ -49,7 +49,14 @@ class PMEnsemble:
             raise ValueError(f"a and da need one value per member ({m})")
         av = (C.c_double * m)(*[float(x) for x in a])
         dv = (C.c_double * m)(*[float(x) for x in da])
-        _lib.call("pm_ensemble_run", self._h, av, dv, int(n_steps))
+        try:
+            _lib.call("pm_ensemble_run", self._h, av, dv, int(n_steps))
+        except RuntimeError as exc:
+            # a member's state became corrupt: the steps done before the check are kept
+            for s in self.members:
+                s.version += 1
+            exc.a_after = list(av)
+            raise
         for s in self.members:
             s.version += 1
         return list(av)

@@ -18,6 +18,7 @@ spectrum of the final density and, on request, the final particle state.
 from __future__ import annotations
 
 import concurrent.futures as cf
+import re
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -84,6 +85,7 @@ class SimResult:
     pk: np.ndarray
     positions: np.ndarray | None = None
     velocities: np.ndarray | None = None
+    error: str | None = None     # set when the request could not be served
 
     @property
     def time_to_completion(self) -> float:
@@ -137,10 +139,17 @@ class PMServer:
     device memory free at construction).  chunk_steps: most steps per batched call; a chunk
     also ends when the first active simulation finishes, so completions and admissions happen
     at chunk boundaries.
+
+    Admission (device allocation, initial conditions, first deposit) and retirement (power
+    spectrum, state copy, release) run in worker threads on each simulation's own stream while
+    the ensemble's chunk runs on its stream, so the device does not idle between chunks.  A
+    request whose start fails, or whose state turns non-finite, ends with ``error`` set; the
+    others are served normally.
     """
 
     def __init__(self, hbm_budget: int | None = None, max_members: int = MAX_MEMBERS,
-                 chunk_steps: int = 16, keep_state: bool = False, prepare_workers: int = 2):
+                 chunk_steps: int = 16, keep_state: bool = False, prepare_workers: int = 2,
+                 start=None, ensemble=None, prepare=None):
         if hbm_budget is None:
             hbm_budget = int(device_memory()[0] * 0.9)
         self.hbm_budget = int(hbm_budget)
@@ -148,6 +157,10 @@ class PMServer:
         self.chunk_steps = max(1, int(chunk_steps))
         self.keep_state = keep_state
         self.prepare_workers = prepare_workers
+        # replaceable pieces (tests of the scheduling and failure paths use stand-ins)
+        self._start = start or _start
+        self._ensemble = ensemble or PMEnsemble
+        self._prepare = prepare or _prepare
         self.stats = {}
 
     def serve(self, requests, realtime: bool = True, clock=time.perf_counter,
@@ -162,88 +175,175 @@ class PMServer:
         pending = deque(reqs)
         queue: list[SimRequest] = []
         prepared: dict[int, cf.Future] = {}
+        starting: list[tuple[SimRequest, cf.Future, float, int]] = []
+        retiring: list[tuple[_Active, cf.Future]] = []
         active: list[_Active] = []
         results: list[SimResult] = []
         used = 0
         busy = admit_s = retire_s = 0.0
         chunks = 0
         peak_members = 0
-        ens = PMEnsemble()
+        failed = 0
+        ens = self._ensemble()
         pool = cf.ThreadPoolExecutor(max_workers=self.prepare_workers)
+        starter = cf.ThreadPoolExecutor(max_workers=1)
+        retirer = cf.ThreadPoolExecutor(max_workers=1)
+        keep = self.keep_state
         t0 = clock()
 
         def now():
             return clock() - t0
 
+        def arrival(req):
+            return req.arrival if realtime else 0.0
+
+        def finish(res):
+            results.append(res)
+            if on_result:
+                on_result(res)
+
+        def fail_result(req, admitted, exc):
+            return SimResult(req.id, arrival(req), admitted, now(), 0, req.a_init,
+                             np.zeros(0), np.zeros(0), error=f"{type(exc).__name__}: {exc}")
+
+        def start_job(req, fut):
+            ta = clock()
+            sim = self._start(req, fut.result())
+            return sim, clock() - ta
+
+        def retire_job(x):
+            tr = clock()
+            try:
+                k, pk, _ = x.sim.power_spectrum(x.req.box)
+                pos = vel = None
+                if keep:
+                    pos, vel = x.sim.get_state()
+            finally:
+                x.sim.close()
+            return k, pk, pos, vel, clock() - tr
+
+        def collect_retired(wait):
+            nonlocal used, retire_s
+            for x, fut in list(retiring):
+                if not (wait or fut.done()):
+                    continue
+                retiring.remove((x, fut))
+                used -= x.fp
+                try:
+                    k, pk, pos, vel, dt = fut.result()
+                    retire_s += dt
+                    finish(SimResult(x.req.id, arrival(x.req), x.admitted, now(), x.req.loop_steps,
+                                     x.a, k, pk, pos, vel))
+                except Exception as exc:  # noqa: BLE001 - reported in the result
+                    finish(fail_result(x.req, x.admitted, exc))
+
+        def join_started(wait):
+            nonlocal used, admit_s, failed
+            for item in list(starting):
+                r, fut, t_adm, fp = item
+                if not (wait or fut.done()):
+                    continue
+                starting.remove(item)
+                try:
+                    sim, dt = fut.result()
+                except Exception as exc:  # noqa: BLE001 - reported in the result
+                    used -= fp
+                    failed += 1
+                    finish(fail_result(r, t_adm, exc))
+                    continue
+                try:
+                    ens.add(sim)
+                except Exception as exc:  # noqa: BLE001
+                    sim.close()
+                    used -= fp
+                    failed += 1
+                    finish(fail_result(r, t_adm, exc))
+                    continue
+                admit_s += dt
+                active.append(_Active(r, sim, r.a_init, r.loop_steps, t_adm, fp))
+
         try:
-            while pending or queue or active:
+            while pending or queue or active or starting or retiring:
                 t = now()
                 while pending and (not realtime or pending[0].arrival <= t):
                     r = pending.popleft()
                     queue.append(r)
-                    prepared[r.id] = pool.submit(_prepare, r)
-                # admission: arrival order, skipping requests that do not fit yet
+                    prepared[r.id] = pool.submit(self._prepare, r)
+                # admission: arrival order, skipping requests that do not fit yet; the sims are
+                # built in the starter thread and join the ensemble at the next chunk boundary
                 for r in list(queue):
-                    if len(active) >= self.max_members:
+                    if len(active) + len(starting) >= self.max_members:
                         break
                     fp = r.footprint()
                     if used + fp > self.hbm_budget:
                         continue
-                    ta = clock()
-                    sim = _start(r, prepared.pop(r.id).result())
-                    ens.add(sim)
-                    admit_s += clock() - ta
-                    active.append(_Active(r, sim, r.a_init, r.loop_steps, now(), fp))
                     used += fp
                     queue.remove(r)
+                    starting.append((r, starter.submit(start_job, r, prepared.pop(r.id)), now(), fp))
+                join_started(wait=not active)
+                collect_retired(wait=not active and not starting)
                 peak_members = max(peak_members, len(active))
                 if not active:
-                    if pending and realtime:
+                    if pending and realtime and not starting:
                         time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
                     continue
                 chunk = min(self.chunk_steps, min(x.remaining for x in active))
                 if chunk > 0:
                     tb = clock()
-                    a_new = ens.run([x.a for x in active], [x.req.da for x in active], chunk)
+                    sick = None
+                    try:
+                        a_new = ens.run([x.a for x in active], [x.req.da for x in active], chunk)
+                    except RuntimeError as exc:
+                        a_new = getattr(exc, "a_after", None)
+                        m = re.search(r"ensemble member (\d+) \(after (\d+) steps\)", str(exc))
+                        if a_new is None or m is None:
+                            raise
+                        sick, chunk = (int(m.group(1)), exc), int(m.group(2))
                     busy += clock() - tb
                     chunks += 1
                     for x, a in zip(active, a_new):
                         x.a = a
                         x.remaining -= chunk
+                    if sick is not None:
+                        x = active[sick[0]]
+                        ens.remove(x.sim)
+                        x.sim.close()
+                        used -= x.fp
+                        active.remove(x)
+                        failed += 1
+                        finish(fail_result(x.req, x.admitted, sick[1]))
                 for x in [x for x in active if x.remaining <= 0]:
-                    tr = clock()
-                    k, pk, _ = x.sim.power_spectrum(x.req.box)
-                    pos = vel = None
-                    if self.keep_state:
-                        pos, vel = x.sim.get_state()
                     ens.remove(x.sim)
-                    x.sim.close()
-                    used -= x.fp
                     active.remove(x)
-                    res = SimResult(x.req.id, x.req.arrival if realtime else 0.0, x.admitted,
-                                    now(), x.req.loop_steps, x.a, k, pk, pos, vel)
-                    results.append(res)
-                    retire_s += clock() - tr
-                    if on_result:
-                        on_result(res)
+                    retiring.append((x, retirer.submit(retire_job, x)))
         finally:
             for x in active:
                 x.sim.close()
+            for _, fut, _, _ in starting:
+                try:
+                    fut.result()[0].close()
+                except Exception:  # noqa: BLE001 - closing what did start
+                    pass
+            starter.shutdown(wait=True)
+            retirer.shutdown(wait=True)
             ens.close()
             pool.shutdown(wait=True)
         wall = now()
         self.stats = {"wall_s": wall, "device_busy_s": busy, "admit_s": admit_s,
                       "retire_s": retire_s, "chunks": chunks, "peak_members": peak_members,
-                      "requests": len(results)}
+                      "requests": len(results), "failed": failed}
         return sorted(results, key=lambda r: r.id)
 
 
-def run_alone(req: SimRequest, keep_state: bool = False) -> SimResult:
+def run_alone(req: SimRequest, keep_state: bool = False, tiles=None) -> SimResult:
     """One request by itself: its own sim, device initial conditions, the graph-replayed
-    step loop and the final power spectrum (the one-at-a-time way of serving)."""
+    step loop and the final power spectrum (the one-at-a-time way of serving).  ``tiles``
+    fixes the tile shape (the ensemble's 16^3 makes the run use the served kernels)."""
     t0 = time.perf_counter()
     sim = _start(req, _prepare(req))
     try:
+        if tiles:
+            sim.set_tiles(*tiles)
         n = req.loop_steps
         a = sim.run(req.a_init, req.da, n)
         k, pk, _ = sim.power_spectrum(req.box)

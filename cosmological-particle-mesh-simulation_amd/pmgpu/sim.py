@@ -21,6 +21,7 @@ from . import _lib
 class PMSim:
     def __init__(self, n_cells: int, n_parts: int, precision: int = 32, *,
                  cic_deconvolve: bool = False, fix_f_cosmology: bool = False, graph: bool = True,
+                 separate_deposit: bool = False, deterministic: bool = False,
                  omega_m0: float = 0.31, omega_lambda0: float = 0.69, omega_k0: float = 0.0,
                  h0: float = 0.68):
         _lib.init()
@@ -35,6 +36,10 @@ class PMSim:
             flags |= _lib.PM_FLAG_FIX_F_COSMOLOGY
         if not graph:
             flags |= _lib.PM_FLAG_NO_GRAPH
+        if separate_deposit:
+            flags |= _lib.PM_FLAG_SEPARATE_DEPOSIT
+        if deterministic:
+            flags |= _lib.PM_FLAG_DETERMINISTIC
         h = C.c_void_p()
         _lib.call("pm_sim_create", self.n_cells, self.n_parts, self.precision, flags, C.byref(h))
         self._h = h
@@ -147,6 +152,14 @@ class PMSim:
         _lib.call("pm_sim_get_density", self._h, out.ctypes.data, dt.itemsize, 0)
         return out
 
+    def density_device(self):
+        """A float32 device copy of the current density grid (a DeviceArray of its own)."""
+        from .device import DeviceArray
+        n = self.n_cells
+        out = DeviceArray((n, n, n), np.float32)
+        _lib.call("pm_sim_get_density", self._h, out.ptr, 4, 1)
+        return out
+
     def potential(self, dtype=np.float32) -> np.ndarray:
         dt = np.dtype(dtype)
         n = self.n_cells
@@ -231,18 +244,41 @@ class SimVelocities(_SimView):
 
 
 class SimDensity:
-    """The sim's current density grid, as returned by density(); copies to the host on demand."""
+    """The density a sim holds at one version of its particles, as returned by density().
+
+    It reads the sim's density buffer on demand while the sim is still at that version.  The
+    next step overwrites that buffer, so a caller that will read the density after stepping
+    (a snapshot or plot of the step's density) calls ``keep()`` first, which takes a device
+    copy; reading a density that was neither kept nor current raises.
+    """
 
     def __init__(self, sim: PMSim, version: int):
         self.sim = sim
         self.version = version
+        self._kept = None
         n = sim.n_cells
         self.shape = (n, n, n)
         self.dtype = np.dtype(np.float32)
         self.ndim = 3
 
+    @property
+    def current(self) -> bool:
+        return self.sim.version == self.version
+
+    def keep(self) -> None:
+        if self._kept is None:
+            if not self.current:
+                raise RuntimeError("this density belongs to an earlier state of the simulation")
+            self._kept = self.sim.density_device()
+
     def __array__(self, dtype=None, copy=None):
-        a = self.sim.density(np.float32)
+        if self._kept is not None:
+            a = self._kept.numpy()
+        elif self.current:
+            a = self.sim.density(np.float32)
+        else:
+            raise RuntimeError("the simulation has stepped since this density was taken and it "
+                               "was not kept (SimDensity.keep) before the step")
         return a if dtype is None else a.astype(dtype)
 
     def __getitem__(self, idx):

@@ -33,12 +33,21 @@ extern "C" {
 #define PM_E_NOMEM 3
 #define PM_E_STATE 4
 #define PM_E_UNSUPPORTED 5
+#define PM_E_CORRUPT 6      /* a step or set_state found non-finite particle values or tile bins
+                               inconsistent with the positions; the message gives the counts   */
 
 /* sim flags */
 #define PM_FLAG_CIC_DECONVOLVE 1u   /* divide by the CIC window of deposit and gather        */
 #define PM_FLAG_FIX_F_COSMOLOGY 2u  /* f() gets [Omega_m, OL, OK] instead of [H0, OL, OK]     */
 #define PM_FLAG_NO_GRAPH 4u         /* pm_sim_run enqueues every kernel instead of replaying a
                                        captured graph of consecutive steps                     */
+#define PM_FLAG_SEPARATE_DEPOSIT 8u /* FP32: deposit into the density grid and run the R2C x
+                                       pass on it instead of the fused row-block kernel that
+                                       transforms the deposit in LDS (the default where the grid
+                                       size and tile shape allow it)                           */
+#define PM_FLAG_DETERMINISTIC 16u   /* deposit with fixed-point LDS sums whose rounding does not
+                                       depend on the order of the particles: repeated runs from
+                                       one state give bit-identical results                    */
 
 /* ---------------------------------------------------------------- library / device memory */
 PM_API int pm_init(int device);                 /* select device, create the library stream */
@@ -118,14 +127,18 @@ PM_API int pm_sim_set_density(pm_sim* sim, const void* rho, int dtype_bytes, int
 /* copy the current density / potential grid out (float32 or float64, host or device)       */
 PM_API int pm_sim_get_density(pm_sim* sim, void* rho, int dtype_bytes, int on_device);
 PM_API int pm_sim_get_potential(pm_sim* sim, void* phi, int dtype_bytes, int on_device);
-/* one PM step: pmesh.py:60-61 (density + advance_time) at scale factor a with step da.     */
+/* one PM step: pmesh.py:60-61 (density + advance_time) at scale factor a with step da.
+ * Returns PM_E_CORRUPT when the step produced non-finite positions or velocities or found its
+ * tile bins inconsistent (the kernels count such particles; the reference carries NaN on
+ * silently).                                                                                 */
 PM_API int pm_sim_step(pm_sim* sim, double a, double da);
 /* n_steps steps from a with step da; returns the final a.  Unless PM_FLAG_NO_GRAPH or timing
  * is set, the steps replay one captured graph of consecutive steps whose scalars (Green's
  * coefficient, kick, drift) come from a device table filled for the whole call.             */
 PM_API int pm_sim_run(pm_sim* sim, double a, double da, int n_steps, double* a_out);
 /* per-stage timing with HIP events on the sim's stream (0 off, 1 on); stage order:
- * deposit, r2c_x, c2c_y_fwd, z_green, c2c_y_inv, c2r_x, gather_kick_drift, bin              */
+ * deposit, r2c_x, c2c_y_fwd, z_green, c2c_y_inv, c2r_x, gather_kick_drift, bin, deposit_r2c
+ * (the fused deposit + R2C x pass, which replaces the first two where it runs)               */
 PM_API int pm_sim_set_timing(pm_sim* sim, int enable);
 PM_API int pm_sim_get_timing(pm_sim* sim, double* stage_ms, int64_t* stage_calls, int n_stages);
 PM_API int pm_sim_sync(pm_sim* sim);
@@ -177,7 +190,9 @@ PM_API int pm_ensemble_add(pm_ensemble* ens, pm_sim* sim);
 PM_API int pm_ensemble_remove(pm_ensemble* ens, pm_sim* sim);
 PM_API int pm_ensemble_size(pm_ensemble* ens, int* n_members);
 /* n_steps steps of every member: member i (in the order of adding) starts at a[i] with step
- * da[i] (pmesh.py:56-63 per member); a[i] is advanced in place.                             */
+ * da[i] (pmesh.py:56-63 per member); a[i] is advanced in place.  PM_E_CORRUPT names the first
+ * member whose state became non-finite ("ensemble member i (after k steps): ..."); the steps
+ * done up to then are kept and a[] reflects them.                                           */
 PM_API int pm_ensemble_run(pm_ensemble* ens, double* a, const double* da, int n_steps);
 /* per-stage timing of the batched launches (same stage order as pm_sim_get_timing); while on,
  * steps are enqueued one by one instead of replaying the graph                               */

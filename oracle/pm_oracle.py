@@ -39,7 +39,8 @@ __all__ = [
     "potential_real", "potential", "f", "H", "Dt", "cell_centers", "weights",
     "sweep_one_direction", "integrate", "advance_time", "step_count", "OracleParams",
     "philox4x32_10", "white_noise",
-    "run", "lattice_q", "power_spectrum",
+    "run", "lattice_q", "power_spectrum", "shaped_field", "zeldovich_displacement",
+    "zeldovich_positions",
 ]
 
 
@@ -316,6 +317,66 @@ def run(positions, velocities, p: OracleParams, n_steps: int | None = None,
         a += da
         k += 1
     return x, v, a, k
+
+
+# --------------------------------------------------------------------------------------------
+# Initial conditions: shaped Gaussian field and Zel'dovich displacement (IC side of the step)
+# --------------------------------------------------------------------------------------------
+def _mode_m2(n: int) -> np.ndarray:
+    """Integer |m|^2 of every mode of an n^3 DFT (fftfreq order on each axis)."""
+    m = np.fft.fftfreq(n, d=1.0 / n)
+    return (m[:, None, None] ** 2 + m[None, :, None] ** 2 + m[None, None, :] ** 2).astype(np.int64)
+
+
+def shaped_field(noise: np.ndarray, amp: np.ndarray) -> np.ndarray:
+    """Density contrast on an n^3 lattice from white noise: IDFT(amp[|m|^2] * DFT(noise)).
+
+    gaussian_random_field.py:9-29 multiplies the transformed field by sqrt(P(k)) per mode; the
+    amplitude is tabulated per integer |m|^2 (amp[0] multiplies the mean).  ``noise`` is
+    (n, n, n) or flat in [i][j][k] order; returns float64 (n, n, n).
+    """
+    w = np.asarray(noise, dtype=np.float64)
+    n = round(w.size ** (1 / 3))
+    w = w.reshape(n, n, n)
+    return np.fft.ifftn(np.fft.fftn(w) * np.asarray(amp, np.float64)[_mode_m2(n)]).real
+
+
+def zeldovich_displacement(delta: np.ndarray, n_cells: int, box: float) -> np.ndarray:
+    """Psi (3, n^3) in grid cells from the density contrast delta on the n^3 particle lattice.
+
+    zeldovich.py:17-69: phi_k = delta_k / (-k^2) (0 at k = 0, :35-38), displacement along
+    direction d = IDFT(-i k_d phi_k) (:56-69 and the backward transform of :45-54) times the
+    force resolution N_CELLS/BOX (:46,53), with k = 2 pi fftfreq(n) n / BOX on array axis d
+    (:60-67), so positions[d] moves along axis d of delta (Q8).  The Nyquist plane of axis d
+    is zeroed: i k_d at k_d = -pi has no real-valued counterpart, and FFTW's real part of it
+    depends on the transform's layout.  Psi satisfies div Psi = -delta.
+    """
+    d = np.asarray(delta, dtype=np.float64)
+    n = d.shape[0]
+    dk = np.fft.fftn(d)
+    k = 2 * np.pi * np.fft.fftfreq(n, d=1.0 / n) / box
+    kd = (k[:, None, None], k[None, :, None], k[None, None, :])
+    k2 = kd[0] ** 2 + kd[1] ** 2 + kd[2] ** 2
+    phi = np.zeros_like(dk)
+    np.divide(dk, -k2, out=phi, where=k2 != 0)
+    out = np.empty((3, n ** 3))
+    for a in range(3):
+        comp = -1j * np.broadcast_to(kd[a], dk.shape) * phi
+        if n % 2 == 0:
+            sl = [slice(None)] * 3
+            sl[a] = n // 2
+            comp[tuple(sl)] = 0.0
+        out[a] = np.fft.ifftn(comp).real.reshape(-1) * (n_cells / box)
+    return out
+
+
+def zeldovich_positions(psi: np.ndarray, n_cells: int) -> np.ndarray:
+    """(q + Psi) mod N_CELLS on the lattice of zeldovich.py:79-87, float64, wrapped into
+    [0, N).  The reference's unseeded +-2 cell jitter (zeldovich.py:89-91, Q10) is left out."""
+    n_parts = round(psi.shape[1] ** (1 / 3))
+    x = np.mod(lattice_q(n_parts, n_cells, np.float64) + psi, n_cells)
+    x[x >= n_cells] = 0.0
+    return x
 
 
 # --------------------------------------------------------------------------------------------

@@ -77,16 +77,27 @@ def f(a, c):
 
 
 def step(pos, vel, fgrid, mass, a, da, om, ol, ok, h0, workers, timings=None):
-    """One PM step in place (pmesh.py:60-61); returns per-stage seconds if ``timings`` is a dict."""
+    """One PM step in place (pmesh.py:60-61).  If ``timings`` is a dict, the seconds of each
+    stage are added to it: density (density.py), fft_forward, green (potential_k), fft_inverse
+    (potential.py:17-29, with the complex128 conversions) and integrate (integrate.py)."""
+    import scipy.fft as sf
+    lib = load()
     n = fgrid.shape[0]
     t0 = time.perf_counter()
     rho = density(pos, mass, n)
     t1 = time.perf_counter()
-    phi = potential(rho, fgrid, a, om, workers)
+    spec = sf.fftn(rho.astype(np.complex128), workers=workers, overwrite_x=True)     # :19-21
     t2 = time.perf_counter()
-    integrate(pos, vel, a, f(a + da, [h0, ol, ok]), da, phi)
+    lib.port_potential_k(spec.ctypes.data, np.ascontiguousarray(fgrid, np.float32).ctypes.data,
+                         spec.size, -3 * om / 8 / a)                                 # :15
     t3 = time.perf_counter()
+    phi = sf.ifftn(spec, workers=workers, overwrite_x=True).real.astype(np.float32)  # :26-29
+    del spec
+    t4 = time.perf_counter()
+    integrate(pos, vel, a, f(a + da, [h0, ol, ok]), da, phi)
+    t5 = time.perf_counter()
     if timings is not None:
-        for k, v in (("density", t1 - t0), ("potential", t2 - t1), ("integrate", t3 - t2)):
+        for k, v in (("density", t1 - t0), ("fft_forward", t2 - t1), ("green", t3 - t2),
+                     ("fft_inverse", t4 - t3), ("integrate", t5 - t4)):
             timings[k] = timings.get(k, 0.0) + v
     return rho, phi

@@ -82,6 +82,54 @@ def test_pmesh_loop_matches_direct_sim_and_writes_snapshots(config):
     assert z["density"].shape == (c.N_CELLS,) * 3
 
 
+def test_snapshot_density_is_the_density_of_its_step(config):
+    # pmesh.py:60-66 writes the density that the step just solved for (deposited from the
+    # pre-step positions), not the one deposited from the positions the step produced
+    from pmgpu.sim import PMSim
+    c = config
+    _run()
+    da = (c.A_END - c.A_INIT) / c.STEPS
+    da_save = (c.A_END - c.A_INIT) / c.N_SAVE_FILES
+    a, k, k1 = c.A_INIT, 0, None
+    while a < c.A_END - da and k1 is None:      # the step after which snapshot 1 is written
+        a += da
+        k += 1
+        if a >= c.A_INIT + 1 * da_save:
+            k1 = k
+    assert k1 is not None and k1 >= 2
+    snap = np.load(f"{c.DATA_DIR}/data.1.npz", allow_pickle=False)["density"]
+    p0, v0 = _ics(c)
+    sim = PMSim(c.N_CELLS, c.N_PARTS, 32, omega_m0=c.OMEGA_M0, omega_lambda0=c.OMEGA_LAMBDA0,
+                omega_k0=c.OMEGA_K0, h0=c.H0)
+    sim.set_state(p0, v0)
+    sim.set_mass((c.N_CELLS / c.N_PARTS) ** 3)
+    sim.run(c.A_INIT, da, k1 - 1)
+    rho_before = sim.density(np.float64)
+    sim.step(c.A_INIT + (k1 - 1) * da, da)
+    rho_after = sim.density(np.float64)
+    sim.close()
+    assert rel(snap, rho_before) < 1e-5
+    assert rel(snap, rho_after) > 1e-3       # the two candidates are far apart at this size
+
+
+def test_stale_sim_density_raises(config):
+    from density import density
+    from pmgpu.sim import PMSim
+    c = config
+    sim = PMSim(c.N_CELLS, c.N_PARTS, 32)
+    p0, v0 = _ics(c)
+    sim.set_state(p0, v0)
+    rho = density(sim.views()[0], (c.N_CELLS / c.N_PARTS) ** 3)
+    kept = density(sim.views()[0], (c.N_CELLS / c.N_PARTS) ** 3)
+    kept.keep()
+    before = np.asarray(rho).copy()
+    sim.step(0.02, 0.01)
+    with pytest.raises(RuntimeError):
+        np.asarray(rho)
+    np.testing.assert_array_equal(np.asarray(kept), before)
+    sim.close()
+
+
 def test_pmesh_restart_continues_from_snapshot(config):
     # restart (pmesh.py:38-42) from snapshot 1 and compare with the resident sim stepped the
     # same way from the same file

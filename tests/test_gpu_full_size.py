@@ -9,7 +9,19 @@ not need the float64 oracle to run at that size:
 * bit-exact particle-to-cell assignment of the reference-layout entry point (integrate.py:20);
 * the potential of the FP32 and FP64 steps against the same solve done with ROCm's FFT library;
 * the whole 1000-step run: P(k) of FP32 against FP64 within 0.5 % for k < k_Nyquist/2.
+
+Against the CPU port of the reference (oracle/pm_port: density.py / integrate.py loop for loop
+with the reference's float32 storage and complex128 FFTs):
+
+* one step at the full size from the same state: density, potential and accelerations of the
+  FP32 and FP64 steps within 1e-5 norm-wise;
+* the whole run from the host (NumPy) initial conditions of seed 7: P(k) of the FP32 and FP64
+  runs within 0.5 % of the port's for k < k_Nyquist/2, the port's run being the committed
+  fixture tests/golden/c2_pk.npz (written by tests/golden/make_c2_pk.py; its initial
+  conditions' checksums are checked first).
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -182,3 +194,56 @@ def test_full_size_run_power_spectrum_fp32_vs_fp64(ics):
     print(f"P(k) FP32 vs FP64 after 1000 steps (a = {a:.4f}): max rel err {err:.2e}")
     assert abs(a - 1.0) < 1e-9
     assert err < 5e-3
+
+
+def test_full_size_step_matches_cpu_port(ics):
+    # density.py:7-48, potential.py:7-29 and integrate.py:9-97 through the port, one step
+    import pm_oracle as O
+    import pm_port
+    pos, vel = ics
+    workers = max(1, min(pm_port.cores(), 16))
+    pm_port.load().port_set_threads(workers)
+    mass = (N / NP) ** 3
+    x, v = pos.copy(), vel.copy()
+    rho_c, phi_c = pm_port.step(x, v, O.fourier_grid(N, ref_dtypes=True), mass, A0, DA,
+                                0.3, 0.7, 0.0, 0.7, workers)
+    acc_c = v.astype(np.float64) - vel
+    for prec in (32, 64):
+        s = _sim(prec)
+        s.set_state(pos, vel)
+        _, v0 = s.get_state(np.float64)
+        rho = s.density(np.float64)
+        s.step(A0, DA)
+        phi = s.potential(np.float64)
+        _, v1 = s.get_state(np.float64)
+        s.close()
+        e_rho, e_phi, e_acc = rel(rho, rho_c), rel(phi, phi_c), rel(v1 - v0, acc_c)
+        print(f"512^3 FP{prec} step vs CPU port: rho {e_rho:.2e} phi {e_phi:.2e} accel {e_acc:.2e}")
+        assert e_rho < 1e-5 and e_phi < 1e-5 and e_acc < 1e-5
+
+
+C2_PK = Path(__file__).resolve().parent / "golden" / "c2_pk.npz"
+
+
+def test_full_run_power_spectrum_matches_cpu_port_fixture():
+    from helpers import make_ics
+    fx = np.load(C2_PK, allow_pickle=False)
+    pos, vel = make_ics(N, NP, seed=7, box=BOX)
+    sums = np.concatenate([pos.astype(np.float64).sum(axis=1), vel.astype(np.float64).sum(axis=1)])
+    np.testing.assert_allclose(sums, fx["ic_sums"], rtol=1e-9, atol=1e-6)
+    steps = int(fx["steps"])
+    assert steps == 1000
+    sel = np.arange(1, len(fx["k"]) + 1) <= N // 4   # k < k_Nyquist / 2
+    for prec in (32, 64):
+        s = _sim(prec)
+        s.set_state(pos, vel)
+        a = s.run(A0, DA, steps)
+        k, pk, cnt = s.power_spectrum(BOX)
+        s.close()
+        np.testing.assert_array_equal(cnt, fx["counts"])
+        np.testing.assert_allclose(k, fx["k"], rtol=1e-12)
+        err = float(np.max(np.abs(pk[sel] / fx["pk"][sel] - 1)))
+        print(f"FP{prec} GPU vs CPU port after {steps} steps (a = {a:.6f}): max P(k) rel err "
+              f"{err:.2e} for k < k_Ny/2; largest scale ratio {pk[0] / fx['pk'][0]:.6f}")
+        assert abs(a - float(fx["a_final"])) < 1e-12
+        assert err < 5e-3

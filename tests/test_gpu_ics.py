@@ -2,7 +2,9 @@
 
 * white noise: the device Philox4x32-10 + Box-Muller equals the oracle restatement (whose
   Philox is pinned to the published known-answer vectors); deviates within 1e-13 relative;
-* Zel'dovich: from the same white noise the device generator equals the host generator
+* Zel'dovich: from the same white noise and amplitude table the device generator equals the
+  oracle's restatement (positions 2e-4 cells / velocities 2e-6 at FP32, 1e-9 / 1e-10 at FP64);
+  it also equals the host generator
   (initial_conditions.gaussian_random_field + zeldovich_numpy, float64 FFTs): positions within
   2e-4 cells (float32 positions), velocities 1e-5 norm-wise at FP32 and 1e-9 at FP64 sims;
 * P(k): the device estimator equals the oracle's power_spectrum of the same density grid,
@@ -58,6 +60,32 @@ def test_device_zeldovich_matches_host_generator(n, npart, prec):
     assert dx < 2e-4
     assert ev < (1e-5 if prec == 32 else 1e-6)   # host velocities are float32
     sim.close()
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("n,npart", [(32, 16), (64, 32), (128, 64)])
+def test_device_zeldovich_matches_oracle(n, npart, prec):
+    # the oracle's restatement of gaussian_random_field.py:9-29 (shaped_field) and
+    # zeldovich.py:10-93 (zeldovich_displacement / zeldovich_positions), pinned by plane-wave
+    # closed forms, fed the same noise and amplitude table as the device generator:
+    # s = Psi, positions q + Psi, v = vel_factor Psi
+    import initial_conditions as ic
+    from pmgpu.device import DeviceArray
+    p = ic_params(n, npart, box=float(2 * n), seed=3)
+    noise = np.random.default_rng(7).standard_normal(npart ** 3)
+    delta = O.shaped_field(noise, ic.amplitude_table(p))
+    psi = O.zeldovich_displacement(delta, n, p["box"])
+    pos_o = O.zeldovich_positions(psi, n)
+    vel_o = ic.velocity_factor(p) * psi
+    sim = _sim(n, npart, prec)
+    ic.zeldovich_device(sim, p, noise=DeviceArray.from_numpy(noise))
+    pos, vel = sim.get_state(np.float64)
+    sim.close()
+    dx = np.abs(periodic_diff(pos, pos_o, n)).max()
+    ev = rel(vel, vel_o)
+    print(f"N={n} Np={npart} fp{prec}: max|dx| {dx:.2e} (|Psi| max {np.abs(psi).max():.2f}) vel {ev:.2e}")
+    assert dx < (2e-4 if prec == 32 else 1e-9)
+    assert ev < (2e-6 if prec == 32 else 1e-10)
 
 
 def test_device_noise_field_is_seeded_and_linear():

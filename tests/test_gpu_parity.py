@@ -414,3 +414,129 @@ def test_autotune_keeps_state_and_results():
     assert np.abs(periodic_diff(xa, xb, n)).max() < 1e-4 and rel(va, vb) < 1e-5
     sim.close()
     ref.close()
+
+
+# ------------------------------------------------------------ fused deposit + R2C x pass
+@pytest.mark.parametrize("n,npart,tiles", [(256, 32, (32, 8, 8)), (384, 48, (64, 8, 8)),
+                                           (512, 64, (32, 8, 8))])
+def test_fused_deposit_r2c_matches_separate_passes(n, npart, tiles):
+    # the row-block kernel that deposits density.py:17-47 into LDS and runs the x transform of
+    # potential.py:19-21 from there, against the tile deposit + R2C pass (FP64 LDS sums): both
+    # FP32 steps are judged against the FP64 step from the same state (the FP32 FFT's own
+    # rounding, ~1e-6 in the potential, dominates either), then over ten graph-replayed steps
+    # and with the density read back in the middle of a run
+    from pmgpu.sim import PMSim
+    pos, vel = make_ics(n, npart, seed=41, box=float(n))
+    kw = dict(omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+    fused, sep = PMSim(n, npart, 32, **kw), PMSim(n, npart, 32, separate_deposit=True, **kw)
+    ref = PMSim(n, npart, 64, **kw)
+    for s in (fused, sep):
+        s.set_tiles(*tiles)
+    for s in (fused, sep, ref):
+        s.set_state(pos, vel)
+    a, da = 0.02, 0.0098
+    out = []
+    for s in (fused, sep, ref):
+        _, v0 = s.get_state(np.float64)
+        s.step(a, da)
+        out.append((s.potential(np.float64), s.get_state(np.float64)[1] - v0))
+    ref.close()
+    e_f = (rel(out[0][0], out[2][0]), rel(out[0][1], out[2][1]))
+    e_s = (rel(out[1][0], out[2][0]), rel(out[1][1], out[2][1]))
+    print(f"N={n} {tiles} vs FP64 step: fused phi {e_f[0]:.2e} kick {e_f[1]:.2e}; "
+          f"separate phi {e_s[0]:.2e} kick {e_s[1]:.2e}")
+    assert max(e_f) < 1e-5 and max(e_s) < 1e-5
+    assert e_f[0] < 3 * e_s[0] + 1e-6 and e_f[1] < 3 * e_s[1] + 1e-6
+    for s in (fused, sep):
+        s.run(a + da, da, 10)
+    e_rho = rel(fused.density(np.float64), sep.density(np.float64))
+    (xf, vf), (xs, vs) = fused.get_state(np.float64), sep.get_state(np.float64)
+    dx = np.abs(periodic_diff(xf, xs, n)).max()
+    print(f"  after 11 steps: rho {e_rho:.2e} max|dx| {dx:.2e} v {rel(vf, vs):.2e}")
+    # the trajectories part by ~3e-5 cells, which moves CIC weights (and rho) by about as much
+    assert e_rho < 5e-5 and dx < 1e-4 and rel(vf, vs) < 1e-5
+    # a density read in the middle of a run (the fused path keeps none) leaves the run unchanged
+    for s in (fused, sep):
+        s.run(a + 11 * da, da, 3)
+    assert rel(fused.density(np.float64), sep.density(np.float64)) < 5e-5
+    fused.close()
+    sep.close()
+
+
+def test_fused_deposit_step_matches_oracle():
+    n, npart = 256, 32
+    pos, vel = make_ics(n, npart, seed=43, box=256.0)
+    sim = _sim(n, npart, 32)
+    assert sim.tiles() == (32, 8, 8)             # the FP32 default at this size runs fused
+    sim.set_state(pos, vel)
+    x0, v0 = sim.get_state(np.float64)
+    p = O.OracleParams(n_cells=n, n_parts=npart, omega_m0=0.3, omega_lambda0=0.7, h0=0.7)
+    rho_ref = O.density(x0, p.mass, n)
+    a, da = 0.02, 0.0098
+    _, v1r, phi_ref = O.advance_time(rho_ref, x0, v0, O.fourier_grid(n), a, da, p, return_phi=True)
+    sim.step(a, da)
+    e_phi = rel(sim.potential(np.float64), phi_ref)
+    e_acc = rel(sim.get_state(np.float64)[1] - v0, v1r - v0)
+    x1, v1 = sim.get_state(np.float64)
+    rho1_ref = O.density(x1, p.mass, n)
+    _, v2r, phi2_ref = O.advance_time(rho1_ref, x1, v1, O.fourier_grid(n), a + da, da, p, return_phi=True)
+    sim.step(a + da, da)                          # this step's spectrum came from the fused kernel
+    e_phi2 = rel(sim.potential(np.float64), phi2_ref)
+    e_acc2 = rel(sim.get_state(np.float64)[1] - v1, v2r - v1)
+    print(f"N=256 fused: step 1 phi {e_phi:.2e} accel {e_acc:.2e}; step 2 phi {e_phi2:.2e} accel {e_acc2:.2e}")
+    assert max(e_phi, e_acc, e_phi2, e_acc2) < 1e-5
+    sim.close()
+
+
+# ------------------------------------------------------------------ deterministic deposit
+@pytest.mark.parametrize("n,npart", [(64, 32), (256, 32)])
+def test_deterministic_deposit_is_bitwise_repeatable(n, npart):
+    # density.py:17,37-47 races (its sums depend on thread timing); with fixed-point LDS sums a
+    # run from one state gives identical bits every time, and agrees with the oracle
+    from pmgpu.sim import PMSim
+    pos, vel = make_ics(n, npart, seed=47, box=float(2 * n))
+    kw = dict(omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+    out = []
+    for rep in range(2):
+        s = PMSim(n, npart, 32, deterministic=True, **kw)
+        s.set_state(pos, vel)
+        rho0 = s.density(np.float64)
+        s.run(0.02, 0.0098, 20)
+        out.append((rho0, *s.get_state(np.float32), s.density(np.float32)))
+        s.close()
+    for a_, b_ in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a_, b_)
+    p = O.OracleParams(n_cells=n, n_parts=npart, omega_m0=0.3, omega_lambda0=0.7, h0=0.7)
+    x0 = O.wrap_positions(pos.astype(np.float64), n)
+    assert rel(out[0][0], O.density(x0, p.mass, n)) < 1e-6
+    ref = _sim(n, npart, 32)
+    ref.set_state(pos, vel)
+    ref.run(0.02, 0.0098, 20)
+    xr, vr = ref.get_state(np.float32)
+    assert np.abs(periodic_diff(out[0][1], xr, n)).max() < 1e-4 and rel(out[0][2], vr) < 1e-5
+    ref.close()
+
+
+# ------------------------------------------------------------------ corrupt state detection
+def test_nonfinite_state_is_reported():
+    from pmgpu import _lib
+    n, npart = 64, 32
+    pos, vel = make_ics(n, npart, seed=53)
+    sim = _sim(n, npart, 32)
+    bad = pos.copy()
+    bad[1, 77] = np.nan
+    with pytest.raises(RuntimeError, match=f"code {_lib.PM_E_CORRUPT}.*1 non-finite"):
+        sim.set_state(bad, vel)
+    # a velocity that overflows the drift: the step reports it instead of binning the particle
+    # at a clamped cell silently
+    sim.set_state(pos, vel)
+    hot = vel.copy()
+    hot[0, 5] = 3e38
+    sim.set_state(pos, hot)
+    with pytest.raises(RuntimeError, match="non-finite"):
+        sim.step(0.02, 0.0098)
+    # the counters are cleared once reported: a good state steps normally again
+    sim.set_state(pos, vel)
+    sim.step(0.02, 0.0098)
+    sim.run(0.02 + 0.0098, 0.0098, 9)
+    sim.close()

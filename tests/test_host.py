@@ -131,6 +131,54 @@ def test_chunks_end_at_the_first_completion(fake_device):
     assert FakeEnsemble.log[2] == ([0], 5)
 
 
+def test_a_failed_start_is_reported_and_the_others_are_served(fake_device):
+    # a start that raises (an allocation failure, say) ends that request with an error; its
+    # budget is released and every other request is served
+    def start(req, prepared):
+        if req.id == 1:
+            raise RuntimeError("libpm pm_sim_create failed (code 3): device allocation failed")
+        sim = FakeSim(req)
+        fake_device.append(sim)
+        return sim
+
+    reqs = [_req(i, 64, 40) for i in range(4)]
+    server = PMServer(hbm_budget=2, max_members=8, chunk_steps=16, start=start)
+    out = server.serve(reqs, realtime=False)
+    assert [r.id for r in out] == [0, 1, 2, 3]
+    assert out[1].error and "allocation failed" in out[1].error and out[1].steps == 0
+    assert all(r.error is None and r.steps == reqs[0].loop_steps for r in (out[0], out[2], out[3]))
+    assert server.stats["failed"] == 1
+    assert all(s.closed for s in fake_device)
+
+
+class SickEnsemble(FakeEnsemble):
+    """Member 1 of the first batched call goes non-finite after 3 of its steps."""
+    fired = False
+
+    def run(self, a, da, n_steps):
+        if not SickEnsemble.fired and len(self.members) > 1:
+            SickEnsemble.fired = True
+            out = super().run(a, da, 3)
+            exc = RuntimeError("libpm pm_ensemble_run failed (code 6): ensemble member 1 "
+                               "(after 3 steps): particle state is corrupt")
+            exc.a_after = out
+            raise exc
+        return super().run(a, da, n_steps)
+
+
+def test_a_member_that_turns_non_finite_is_retired_with_an_error(fake_device):
+    SickEnsemble.fired = False
+    reqs = [_req(i, 64, 40) for i in range(3)]
+    server = PMServer(hbm_budget=10, max_members=8, chunk_steps=16, ensemble=SickEnsemble)
+    out = server.serve(reqs, realtime=False)
+    assert out[1].error and "corrupt" in out[1].error
+    for r in (out[0], out[2]):
+        assert r.error is None and r.steps == reqs[0].loop_steps
+        assert r.a_final == _replay(reqs[0])       # the 3 steps before the failure count
+    assert server.stats["failed"] == 1
+    assert all(s.closed for s in fake_device)
+
+
 def test_request_larger_than_the_budget_is_rejected(fake_device):
     with pytest.raises(ValueError, match="HBM budget"):
         PMServer(hbm_budget=10).serve([_req(0, 384, 50)], realtime=False)

@@ -13,7 +13,17 @@ inputs, chosen so that a dropped term, a wrong sign, index or axis order fails o
     weights/sweep/integ.  test_gather_plane_wave, test_zero_self_force, test_momentum,
                           test_pair_attraction, test_free_drift, test_axis_order
     f / advance_time      test_f_*, test_advance_time_uses_h0_slot
+    H                     test_f_is_inverse_aH (H(a) = 1/(a f) with h0 = 1), test_h_closed_form
+    Dt                    test_dt_einstein_de_sitter_and_linear, test_dt_against_cpt92
     step_count            test_step_count
+    run                   test_run_free_particle_closed_form, test_run_counts_and_limit
+    lattice_q             test_lattice_q_explicit
+    wrap_positions        test_deposit_periodic_edge_and_q4, test_wrap_positions_edges
+    shaped_field          test_shaped_field_constant_amplitude, test_shaped_field_one_mode
+    zeldovich_displacement  test_zeldovich_plane_wave_each_axis, test_zeldovich_divergence
+    zeldovich_positions   test_zeldovich_positions_wrap_lattice
+    power_spectrum        test_power_spectrum_of_plane_wave
+    philox4x32_10 / white_noise  test_philox_known_answers, test_white_noise_*
 """
 import math
 
@@ -312,3 +322,153 @@ def test_white_noise_is_standard_normal_and_seeded():
     t = 2 * math.pi * (u[1] + 0.5) * 2.0 ** -32
     assert z[0] == pytest.approx(r * math.cos(t), rel=1e-15)
     assert z[1] == pytest.approx(r * math.sin(t), rel=1e-15)
+
+
+# ----------------------------------------------------------------- growth factor and lattice
+def test_dt_einstein_de_sitter_and_linear():
+    # cosmology.py:29-36 in an Omega_m = 1 universe is the textbook D(a) = a, and linear in a
+    # for any cosmology
+    for a in (0.01, 0.3, 1.0):
+        assert abs(O.Dt(a, [1.0, 0.0, 0.0]) - a) < 1e-15
+    c = [0.3, 0.7, 0.0]
+    assert abs(O.Dt(0.6, c) - 2 * O.Dt(0.3, c)) < 1e-14
+
+
+def test_dt_against_cpt92():
+    # Carroll, Press & Turner (1992, ARA&A 30, 499, eq. 29): g = 5/2 Om / (Om^(4/7) - OL +
+    # (1 + Om/2)(1 + OL/70)).  The reference's expression divides by Omega_m where CPT92
+    # multiplies (cosmology.py:36), i.e. Dt = g a / Om^2
+    for om, ol in ((0.3, 0.7), (0.5, 0.5), (1.0, 0.0), (0.25, 0.0)):
+        g = 2.5 * om / (om ** (4.0 / 7.0) - ol + (1 + om / 2) * (1 + ol / 70))
+        assert abs(O.Dt(0.7, [om, ol, 0.0]) - g * 0.7 / om ** 2) < 1e-13
+
+
+def test_h_closed_form():
+    # cosmology.py:11-18 at a = 1 is h0 for a flat universe, and scales as a^-3/2 for matter
+    assert abs(O.H(1.0, 0.7, [0.3, 0.7, 0.0]) - 0.7) < 1e-15
+    assert abs(O.H(0.25, 2.0, [1.0, 0.0, 0.0]) - 2.0 * 8.0) < 1e-12
+
+
+def test_lattice_q_explicit():
+    # zeldovich.py:79-83 for N_PARTS = 2, N_CELLS = 8: axis = [0, 4] + 0.5, particle
+    # p = (i*2 + j)*2 + k at (q_i, q_j, q_k)
+    q = O.lattice_q(2, 8, np.float64)
+    lo, hi = 0.5, 4.5
+    np.testing.assert_array_equal(q[0], [lo, lo, lo, lo, hi, hi, hi, hi])
+    np.testing.assert_array_equal(q[1], [lo, lo, hi, hi, lo, lo, hi, hi])
+    np.testing.assert_array_equal(q[2], [lo, hi, lo, hi, lo, hi, lo, hi])
+    q3 = O.lattice_q(3, 12, np.float64)    # spacing N_CELLS/N_PARTS = 4, not a power of two
+    np.testing.assert_array_equal(q3[2, :3], [0.5, 4.5, 8.5])
+    np.testing.assert_array_equal(q3[0, ::9], [0.5, 4.5, 8.5])
+    assert O.lattice_q(4, 8).dtype == np.float32
+
+
+def test_wrap_positions_edges():
+    n = 8
+    x = np.array([-1e-9, 0.0, 7.999, 8.0, 11.5, -3.25], np.float64)
+    np.testing.assert_array_equal(O.wrap_positions(x, n), [8 - 1e-9, 0.0, 7.999, 0.0, 3.5, 4.75])
+    tiny = np.array([-1e-9], np.float32)          # rounds to n after the add: mapped to 0
+    assert O.wrap_positions(tiny, n)[0] == 0.0
+
+
+# --------------------------------------------------------------------------------- the loop
+def test_run_free_particle_closed_form():
+    # one particle feels no force (test_zero_self_force), so run() drifts it by
+    # v * sum_k da f(a_k + da) / (a_k + da)^2 with f of [H0, OL, OK] (integrate.py:12,95) over
+    # the pmesh.py:56-63 steps
+    n = 8
+    p = O.OracleParams(n_cells=n, n_parts=1, omega_m0=0.3, omega_lambda0=0.7, h0=0.7,
+                       a_init=0.1, a_end=1.0, steps=12)
+    x0 = np.array([[2.3], [5.1], [7.6]])
+    v0 = np.array([[0.4], [-0.25], [0.1]])
+    x, v, a, k = O.run(x0, v0, p)
+    da = 0.9 / 12
+    kk = O.step_count(0.1, 1.0, 12)
+    ak = 0.1 + da * np.arange(kk) + da                         # a + da of each step
+    fa = 1 / np.sqrt((0.7 + 0.7 * ak ** 3) / ak)
+    drift = np.sum(da * fa / ak ** 2)
+    assert k == kk and abs(a - (0.1 + kk * da)) < 1e-12
+    np.testing.assert_allclose(v, v0, atol=1e-12)
+    np.testing.assert_allclose(x, np.mod(x0 + v0 * drift, n), atol=1e-10)
+
+
+def test_run_counts_and_limit():
+    n = 8
+    p = O.OracleParams(n_cells=n, n_parts=2, omega_m0=0.3, omega_lambda0=0.7, h0=0.7,
+                       a_init=0.02, a_end=1.0, steps=100)
+    x0 = O.lattice_q(2, n, np.float64) + 0.3
+    v0 = np.zeros_like(x0)
+    _, _, a, k = O.run(x0, v0, p, n_steps=5)
+    assert k == 5 and abs(a - (0.02 + 5 * 0.0098)) < 1e-12
+    *_, k_all = O.run(x0, v0, p)
+    assert k_all == 99                     # Q7: STEPS = 100 from 0.02 runs 99 steps
+
+
+# ----------------------------------------------------------------- initial conditions
+def test_shaped_field_constant_amplitude():
+    # amp = c for every |m|^2 >= 1 and amp[0] = 0: the field is c times the noise minus its mean
+    n = 8
+    w = rng(20).normal(size=(n, n, n))
+    amp = np.full(3 * (n // 2) ** 2 + 1, 1.7)
+    amp[0] = 0.0
+    np.testing.assert_allclose(O.shaped_field(w, amp), 1.7 * (w - w.mean()), atol=1e-12)
+
+
+def test_shaped_field_one_mode():
+    # a plane wave of mode m = (1, 2, 0) has |m|^2 = 5: only amp[5] scales it
+    n = 8
+    i = np.arange(n)
+    w = np.cos(2 * np.pi * (1 * i[:, None, None] + 2 * i[None, :, None]) / n) * np.ones((1, 1, n))
+    amp = np.zeros(3 * (n // 2) ** 2 + 1)
+    amp[5] = 3.0
+    np.testing.assert_allclose(O.shaped_field(w.reshape(-1), amp), 3.0 * w, atol=1e-12)
+    amp[5], amp[4] = 0.0, 1.0
+    assert np.abs(O.shaped_field(w, amp)).max() < 1e-12
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_zeldovich_plane_wave_each_axis(axis):
+    # delta = A cos(k q) along lattice axis `axis` (k = 2 pi m / BOX): the Zel'dovich
+    # displacement is Psi = -(A/k) sin(k q) along that axis only (div Psi = -delta), in cells
+    # N_CELLS/BOX per Mpc/h, and moves positions[axis] (Q8)
+    n, n_cells, box, m, A = 8, 16, 40.0, 2, 0.3
+    i = np.arange(n)
+    q = i * box / n
+    k = 2 * np.pi * m / box
+    shape = [1, 1, 1]
+    shape[axis] = n
+    delta = np.broadcast_to((A * np.cos(k * q)).reshape(shape), (n, n, n))
+    psi = O.zeldovich_displacement(delta, n_cells, box)
+    expect = np.broadcast_to((-(A / k) * np.sin(k * q)).reshape(shape), (n, n, n)).reshape(-1)
+    for a in range(3):
+        np.testing.assert_allclose(psi[a], expect * (n_cells / box) if a == axis else 0.0,
+                                   atol=1e-12)
+
+
+def test_zeldovich_divergence():
+    # a sum of two waves, one oblique: delta = A cos(k.q + p) gives Psi = -A k/|k|^2 sin(k.q + p)
+    # and B sin(k2 x) gives Psi_x = (B/k2) cos(k2 x); both satisfy div Psi = -delta
+    n, n_cells, box = 8, 8, 8.0        # one cell per Mpc/h, so Psi is in the field's units
+    i = np.arange(n)
+    z, y, x = np.meshgrid(i, i, i, indexing="ij")
+    delta = 0.2 * np.cos(2 * np.pi * (z + 2 * y - x) / n + 0.3) + 0.1 * np.sin(2 * np.pi * 3 * x / n)
+    psi = O.zeldovich_displacement(delta, n_cells, box).reshape(3, n, n, n)
+    kz, ky, kx = (2 * np.pi / n) * np.array([1, 2, -1])
+    kk = kz ** 2 + ky ** 2 + kx ** 2
+    ph = 2 * np.pi * (z + 2 * y - x) / n + 0.3
+    expect_z = -0.2 * kz / kk * np.sin(ph)
+    k2 = 2 * np.pi * 3 / n
+    expect_x = -0.2 * kx / kk * np.sin(ph) + 0.1 / k2 * np.cos(k2 * x)
+    np.testing.assert_allclose(psi[0], expect_z, atol=1e-12)
+    np.testing.assert_allclose(psi[2], expect_x, atol=1e-12)
+
+
+def test_zeldovich_positions_wrap_lattice():
+    n_parts, n_cells = 2, 8
+    psi = np.zeros((3, 8))
+    psi[0, 0] = -1.0          # 0.5 - 1 wraps to 7.5
+    psi[2, 7] = 3.5           # 4.5 + 3.5 = 8 wraps to 0
+    x = O.zeldovich_positions(psi, n_cells)
+    q = O.lattice_q(n_parts, n_cells, np.float64)
+    assert x[0, 0] == 7.5 and x[2, 7] == 0.0
+    np.testing.assert_array_equal(x[1], q[1])
