@@ -394,6 +394,9 @@ struct pm_sim {
   void* grid[2] = {nullptr, nullptr};
   int gb = 0;
   void* Q = nullptr;
+  // FP32, N % 32 == 0: the x margins of phi at every multiple of 32, written by the C2R pass for
+  // the gather's 32-wide tiles (N^2 * N/32 float4)
+  float4* edge = nullptr;
   const float* fgrid = nullptr;  // optional caller-owned Green's function grid
   FftTables* tab = nullptr;
   TileGeom G{};
@@ -436,10 +439,14 @@ size_t bins_bytes(int n, int np_side, int prec) {
 // the captured graphs and the health counters (a generous bound)
 constexpr size_t kSimOverhead = 1u << 20;
 
+bool has_edge(int n, int prec) { return prec == 4 && n % 32 == 0 && n >= 64; }
+size_t edge_bytes(int n) { return (size_t)n * n * (n / 32) * sizeof(float4); }
+
 size_t sim_bytes(int n, int np_side, int prec) {
   const size_t nn = (size_t)n, np = (size_t)np_side * np_side * np_side;
   return 2 * (6 * np * prec + 4 * np) /*two state buffers*/ + nn * nn * nn * prec /*rho*/ +
-         nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/ + bins_bytes(n, np_side, prec) + kSimOverhead;
+         nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/ + bins_bytes(n, np_side, prec) + kSimOverhead +
+         (has_edge(n, prec) ? edge_bytes(n) : 0);
 }
 
 template <typename R> PState<R> state_buf(const pm_sim* s, int k) { return {(R*)s->sb[k], (R*)s->vb[k], s->idb[k]}; }
@@ -451,10 +458,13 @@ double det_fix(const pm_sim* s) {
 }
 void* spec_slot(const pm_sim* s) { return s->grid[s->gb ^ 1]; }
 
-// the step's deposit and R2C x pass run as one row-block kernel (FP32, tiles of 8 x 8 cells in
-// y and z, grid sizes with an instantiation) unless the sim asks for the separate passes
+// the step's deposit and R2C x pass run as one row-block kernel when the sim asks for it (FP32,
+// tiles of 8 x 8 cells in y and z, grid sizes with an instantiation).  Not the default: at
+// 512^3 its 137 KB of LDS per row block leave one workgroup per CU, whose load, accumulate,
+// transform and write phases cannot overlap, and it ran 0.90 ms against 0.44 ms for the tile
+// deposit and the R2C pass it replaces.
 bool fused_deposit(const pm_sim* s) {
-  return s->prec == 4 && !(s->flags & (PM_FLAG_SEPARATE_DEPOSIT | PM_FLAG_DETERMINISTIC)) &&
+  return s->prec == 4 && (s->flags & PM_FLAG_FUSED_DEPOSIT) && !(s->flags & PM_FLAG_DETERMINISTIC) &&
          deposit_r2c_supported(s->L.n, s->G);
 }
 
@@ -614,9 +624,11 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
     StageTimer t(s, ST_Y_INV);
     PM_HIP(fft_c2c_y<R>(H, Q, n, +1, *s->tab, s->stream, work));
   }
+  // the gather's 32-wide tiles read their x margins from the edge array the C2R pass writes
+  float4* edge = (s->edge && s->G.tx == 32) ? s->edge : nullptr;
   {
     StageTimer t(s, ST_C2R);
-    PM_HIP((fft_c2r_x<R, R>(H, Q, (R*)H, n, *s->tab, s->stream)));
+    PM_HIP((fft_c2r_x<R, R>(H, Q, (R*)H, n, *s->tab, s->stream, edge)));
   }
   if (in_rho_slot) s->gb ^= 1;  // the potential now sits in the spectrum slot
   s->phi_valid = true;
@@ -629,7 +641,7 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
   {
     StageTimer t(s, ST_GKD);
     PM_HIP(gkd_tiles<R>(cur_state<R>(s), state_buf<R>(s, s->cur ^ 1), (const R*)spec_slot(s), sc.kick, sc.drift,
-                        s->L, s->G, s->B, tab, s->stream));
+                        s->L, s->G, s->B, tab, s->stream, edge));
   }
   s->cur ^= 1;
   {
@@ -881,7 +893,8 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
          hipMalloc(&s->vb[k], 3 * (size_t)s->L.np * pr) == hipSuccess &&
          hipMalloc(&s->idb[k], (size_t)s->L.np * 4) == hipSuccess;
   if (!ok || hipMalloc(&s->grid[0], nn * nn * nn * pr) != hipSuccess ||
-      hipMalloc(&s->grid[1], nn * nn * nn * pr) != hipSuccess || hipMalloc(&s->Q, nn * nn * 2 * pr) != hipSuccess)
+      hipMalloc(&s->grid[1], nn * nn * nn * pr) != hipSuccess || hipMalloc(&s->Q, nn * nn * 2 * pr) != hipSuccess ||
+      (has_edge(n_cells, s->prec) && hipMalloc(&s->edge, edge_bytes(n_cells)) != hipSuccess))
     return cleanup(fail(PM_E_NOMEM, "device allocation of %zu bytes failed", sim_bytes(n_cells, n_parts_side, s->prec)));
   s->G = make_tiles(n_cells, s->prec);
   const TileGeom gmax = max_tiles(n_cells, s->prec);
@@ -925,6 +938,7 @@ PM_API int pm_sim_destroy(pm_sim* s) {
   }
   (void)hipFree(s->grid[0]);
   (void)hipFree(s->grid[1]);
+  (void)hipFree(s->edge);
   (void)hipFree(s->B.health);
   if (s->h_health) (void)hipHostFree(s->h_health);
   (void)hipFree(s->Q);

@@ -285,6 +285,8 @@ template <typename R>
 hipError_t fft_z_green(cpx<R>* H, cpx<R>* Q, int n, const GreenArgs& g, const FftTables& t, hipStream_t st,
                        cpx<R>* work = nullptr);
 template <typename R, typename Tout>
-hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTables& t, hipStream_t st);
+// edge (FP32 only, N % 32 == 0; may be null): also write the x margins of 32-wide tiles, see c2r_body
+hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTables& t, hipStream_t st,
+                     float4* edge = nullptr);
 
 }  // namespace pm

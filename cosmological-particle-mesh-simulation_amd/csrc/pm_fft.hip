@@ -130,10 +130,14 @@ __global__ __launch_bounds__(kThreads) void k_r2c_x(const Tin* __restrict__ rho,
 // ------------------------------------------------------------------------------------------
 // x pass, complex -> real (in place allowed: H and out may be the same memory)
 // ------------------------------------------------------------------------------------------
+// edge (optional, FP32): the x margins of the gather's 32-wide tiles.  For every row (z, y) and
+// every multiple of 32, b * 32, edge[(z * N/32 + b) * N + y] = (phi(32b - 1), phi(32b),
+// phi(32b + 1), 0) (x periodic), so the tile b gather reads its left margin from entry b and its
+// two right margins from entry b + 1 instead of three partial cache lines per potential row.
 template <typename R, typename Tout, int N, int RB>
 __device__ __forceinline__ void c2r_body(const cpx<R>* H, const cpx<R>* __restrict__ Q, Tout* out,
                                          const cpx<R>* __restrict__ twg, long blk, cpx<R>* lds, cpx<R>* tw,
-                                         cpx<R>* qx) {
+                                         cpx<R>* qx, float4* __restrict__ edge = nullptr) {
   using T = cpx<R>;
   constexpr int M = N / 2;
   constexpr int SN = RB + 1;
@@ -198,6 +202,23 @@ __device__ __forceinline__ void c2r_body(const cpx<R>* H, const cpx<R>* __restri
   }
   __syncthreads();
   fft_passes<T, M, 1, +1, RB, SN, NT, 2>(lds, tw, tid);
+  if constexpr (sizeof(R) == 4 && sizeof(Tout) == 4 && N % 32 == 0) {
+    if (edge) {
+      constexpr int NB = N / 32;
+      auto phi_at = [&](int x, int w) {
+        const T v = lds[(x >> 1) * SN + w];
+        return (x & 1) ? v.y : v.x;
+      };
+      for (int i = tid; i < RB * NB; i += NT) {
+        const int w = i % RB, b = i / RB;
+        const size_t row = row0 + w;
+        const size_t z = row / N, y = row - z * N;
+        const int x0 = 32 * b;
+        edge[(z * NB + b) * N + y] = make_float4(phi_at(x0 == 0 ? N - 1 : x0 - 1, w), phi_at(x0, w),
+                                                 phi_at(x0 + 1, w), 0.f);
+      }
+    }
+  }
   cpx<Tout>* dst = reinterpret_cast<cpx<Tout>*>(out) + row0 * M;
 #pragma unroll 4
   for (int e = tid; e < RB * M; e += NT) {
@@ -209,11 +230,11 @@ __device__ __forceinline__ void c2r_body(const cpx<R>* H, const cpx<R>* __restri
 
 template <typename R, typename Tout, int N, int RB>
 __global__ __launch_bounds__(kThreads) void k_c2r_x(const cpx<R>* H, const cpx<R>* __restrict__ Q, Tout* out,
-                                                    const cpx<R>* __restrict__ twg) {
+                                                    const cpx<R>* __restrict__ twg, float4* __restrict__ edge) {
   __shared__ cpx<R> lds[(N / 2) * (RB + 1)];
   __shared__ cpx<R> tw[N];
   __shared__ cpx<R> qx[RB];
-  c2r_body<R, Tout, N, RB>(H, Q, out, twg, blockIdx.x, lds, tw, qx);
+  c2r_body<R, Tout, N, RB>(H, Q, out, twg, blockIdx.x, lds, tw, qx, edge);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -735,9 +756,9 @@ hipError_t launch_r2c(const Tin* rho, cpx<R>* H, cpx<R>* Q, const FftTables& t, 
   return hipGetLastError();
 }
 template <typename R, typename Tout, int N>
-hipError_t launch_c2r(cpx<R>* H, const cpx<R>* Q, Tout* out, const FftTables& t, hipStream_t st) {
+hipError_t launch_c2r(cpx<R>* H, const cpx<R>* Q, Tout* out, const FftTables& t, hipStream_t st, float4* edge) {
   constexpr int RB = rows_per_block<R, N>();
-  k_c2r_x<R, Tout, N, RB><<<(long)N * N / RB, kThreads, 0, st>>>(H, Q, out, (const cpx<R>*)t.tw);
+  k_c2r_x<R, Tout, N, RB><<<(long)N * N / RB, kThreads, 0, st>>>(H, Q, out, (const cpx<R>*)t.tw, edge);
   return hipGetLastError();
 }
 // Page-tiled spectrum for N >= 1024 (FP32).  The plain layout puts consecutive z of a column
@@ -826,9 +847,10 @@ hipError_t fft_r2c_x(const Tin* rho, cpx<R>* H, cpx<R>* Q, int n, const FftTable
   return hipErrorInvalidValue;
 }
 template <typename R, typename Tout>
-hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTables& t, hipStream_t st) {
+hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTables& t, hipStream_t st,
+                     float4* edge) {
   switch (n) {
-#define PM_CASE(NN) case NN: return launch_c2r<R, Tout, NN>(H, Q, out, t, st);
+#define PM_CASE(NN) case NN: return launch_c2r<R, Tout, NN>(H, Q, out, t, st, edge);
     PM_FOR_EACH_N(PM_CASE)
 #undef PM_CASE
   }
@@ -877,12 +899,12 @@ template hipError_t fft_r2c_x<double, double>(const double*, cpx<double>*, cpx<d
                                                hipStream_t);
 template hipError_t fft_r2c_x<double, float>(const float*, cpx<double>*, cpx<double>*, int, const FftTables&,
                                               hipStream_t);
-template hipError_t fft_c2r_x<float, float>(cpx<float>*, const cpx<float>*, float*, int, const FftTables&,
-                                             hipStream_t);
-template hipError_t fft_c2r_x<double, double>(cpx<double>*, const cpx<double>*, double*, int, const FftTables&,
-                                               hipStream_t);
-template hipError_t fft_c2r_x<double, float>(cpx<double>*, const cpx<double>*, float*, int, const FftTables&,
-                                              hipStream_t);
+template hipError_t fft_c2r_x<float, float>(cpx<float>*, const cpx<float>*, float*, int,
+                                                   const FftTables&, hipStream_t, float4*);
+template hipError_t fft_c2r_x<double, double>(cpx<double>*, const cpx<double>*, double*, int,
+                                                   const FftTables&, hipStream_t, float4*);
+template hipError_t fft_c2r_x<double, float>(cpx<double>*, const cpx<double>*, float*, int,
+                                                   const FftTables&, hipStream_t, float4*);
 #define PM_INST(R)                                                                                   \
   template hipError_t fft_c2c_y<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t, cpx<R>*); \
   template hipError_t fft_c2c_z<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t);       \

@@ -150,8 +150,11 @@ hipError_t deposit_r2c(const PState<float>& p, float mass, const Lattice& L, con
 // Reads `in` through the bins' own entries and writes the updated particles to `out` at their
 // own-entry index (never in place); also counts the entries of the new positions into B.counts.
 template <typename R>
+// edge (FP32, 32-wide tiles; may be null): the C2R pass's x-margin array of phi, which lets a
+// box row be read as one aligned cache line
 hipError_t gkd_tiles(const PState<R>& in, const PState<R>& out, const R* phi, double kick, double drift,
-                     const Lattice& L, const TileGeom& G, const Bins& B, const StepTable& tab, hipStream_t st);
+                     const Lattice& L, const TileGeom& G, const Bins& B, const StepTable& tab, hipStream_t st,
+                     const float4* edge = nullptr);
 
 // ---- reference-layout kernels (absolute float32 positions, reference order) ------------------
 hipError_t ref_density(const float* pos, long np, int n, double mass, float* rho, hipStream_t st);

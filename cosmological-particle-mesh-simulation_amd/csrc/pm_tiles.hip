@@ -737,7 +737,7 @@ __device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restr
                                          const Lattice& L, const TileGeom& G,
                                          const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
                                          int tile, uint32_t beg, uint32_t end, const uint32_t (&sl0)[kGkdE], R* box,
-                                         uint32_t* hist) {
+                                         uint32_t* hist, const float4* __restrict__ edge) {
   constexpr int BX = TX + 3, BY = TY + 3, BZ = TZ + 3;
   constexpr int NW = kT / 64;                       // wavefronts per block
   constexpr int PLANE = BX * BY;                    // values per box plane
@@ -781,8 +781,69 @@ __device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restr
   };
   // potential of the tile and its -1/+2 stencil margin (periodic).  Each wavefront loads whole
   // z planes of the box (BY x BX values in PL loads of 64 lanes): a lane's in-plane offsets are
-  // computed once, and a plane adds one wave-uniform base.
-  {
+  // computed once, and a plane adds one wave-uniform base.  For 32-wide FP32 tiles with the
+  // C2R pass's edge array, a box row is its tile's aligned 32 values (one cache line) plus the
+  // three margin values from the edge entries of the tile's two x boundaries.
+  bool staged = false;
+  if constexpr (TX == 32 && sizeof(R) == 4) {
+    if (edge) {
+      staged = true;
+      const int lane = threadIdx.x & 63;
+      const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+      constexpr int MAIN = 32 * BY;  // box values that lie in the tile's own 32 columns
+      const int kb1 = tix + 1 == G.ntx ? 0 : tix + 1;
+      uint32_t off[PL];
+#pragma unroll
+      for (int k = 0; k < PL; ++k) {
+        const int e = k * 64 + lane;
+        if (e < MAIN) {
+          off[k] = (uint32_t)(fold(o0[1] - 1 + (e >> 5), n) * n + o0[0] + (e & 31));
+        } else if (e < PLANE) {
+          const int j = e - MAIN, by = j / 3, c = j - 3 * by;
+          off[k] = (uint32_t)((((c == 0 ? tix : kb1) * n + fold(o0[1] - 1 + by, n)) << 2) + c);
+        } else {
+          off[k] = 0u;
+        }
+      }
+      R tmp[ZIT][PL];
+#pragma unroll
+      for (int it = 0; it < ZIT; ++it) {
+        const int bz = wv + NW * it;
+        if (bz < BZ) {
+          const size_t zf = (size_t)fold(o0[2] - 1 + bz, n);
+          const R* plane = phi + zf * nn * nn;
+          const R* eplane = reinterpret_cast<const R*>(edge) + zf * (size_t)G.ntx * nn * 4;
+#pragma unroll
+          for (int k = 0; k < PL; ++k) {
+            const int e = k * 64 + lane;
+            if (e < PLANE) tmp[it][k] = (e < MAIN ? plane : eplane)[off[k]];
+          }
+        }
+      }
+      load_state();
+#pragma unroll
+      for (int it = 0; it < ZIT; ++it) {
+        const int bz = wv + NW * it;
+        if (bz < BZ) {
+#pragma unroll
+          for (int k = 0; k < PL; ++k) {
+            const int e = k * 64 + lane;
+            if (e < PLANE) {
+              int dst;
+              if (e < MAIN) {
+                dst = (e >> 5) * BX + 1 + (e & 31);
+              } else {
+                const int j = e - MAIN, by = j / 3, c = j - 3 * by;
+                dst = by * BX + (c == 0 ? 0 : 32 + c);
+              }
+              box[bz * PLANE + dst] = tmp[it][k];
+            }
+          }
+        }
+      }
+    }
+  }
+  if (!staged) {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t off[PL];
@@ -937,11 +998,13 @@ template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
 __device__ __forceinline__ void gkd_body(const GkdState<R>& P, const R* __restrict__ phi, R kick, R drift,
                                          const Lattice& L, const TileGeom& G, const uint32_t* __restrict__ offsets,
                                          const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
-                                         uint32_t capacity, int tile, R* box, uint32_t* hist) {
+                                         uint32_t capacity, int tile, R* box, uint32_t* hist,
+                                         const float4* __restrict__ edge = nullptr) {
   uint32_t beg, end, sl[kGkdE];
   tile_range(offsets, capacity, tile, beg, end);
   first_entries(entries, beg, end, L.np, sl);
-  gkd_tile<R, TX, TY, TZ, LOCAL_HIST>(P, phi, kick, drift, L, G, entries, counts, tile, beg, end, sl, box, hist);
+  gkd_tile<R, TX, TY, TZ, LOCAL_HIST>(P, phi, kick, drift, L, G, entries, counts, tile, beg, end, sl, box, hist,
+                                      edge);
 }
 
 // FP32 with the LDS histogram: a register budget of 7 workgroups per CU (72 VGPRs, no spills;
@@ -950,7 +1013,8 @@ template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
 __global__ __launch_bounds__(kT, sizeof(R) == 4 && LOCAL_HIST ? 7 : 1) void k_gkd_tiles(GkdState<R> P, const R* __restrict__ phi, R kick, R drift,
                                                   Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
-                                                  uint32_t capacity, StepTable tab) {
+                                                  uint32_t capacity, StepTable tab,
+                                                  const float4* __restrict__ edge) {
   __shared__ R box[gkd_box_cells<TX, TY, TZ>()];
   __shared__ uint32_t hist[128];
   if (tab.table) {  // graph replay: this step's kick and drift from the device table
@@ -959,7 +1023,7 @@ __global__ __launch_bounds__(kT, sizeof(R) == 4 && LOCAL_HIST ? 7 : 1) void k_gk
     drift = R(tab.table[3 * k + 2]);
   }
   gkd_body<R, TX, TY, TZ, LOCAL_HIST>(P, phi, kick, drift, L, G, offsets, entries, counts, capacity,
-                                      xcd_tile(blockIdx.x, gridDim.x), box, hist);
+                                      xcd_tile(blockIdx.x, gridDim.x), box, hist, edge);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1149,7 +1213,8 @@ hipError_t deposit_tiles(const PState<R>& p, R* rho, double mass, const Lattice&
 
 template <typename R>
 hipError_t gkd_tiles(const PState<R>& in, const PState<R>& out, const R* phi, double kick, double drift,
-                     const Lattice& L, const TileGeom& G, const Bins& B, const StepTable& tab, hipStream_t st) {
+                     const Lattice& L, const TileGeom& G, const Bins& B, const StepTable& tab, hipStream_t st,
+                     const float4* edge) {
   if (in.s == out.s || in.v == out.v || in.id == out.id) return hipErrorInvalidValue;  // never in place
   const bool local_hist = G.ntx >= 4 && G.nty >= 4 && G.ntz >= 4;
   const GkdState<R> ps{in.s, in.v, in.id, out.s, out.v, out.id};
@@ -1157,10 +1222,10 @@ hipError_t gkd_tiles(const PState<R>& in, const PState<R>& out, const R* phi, do
   do {                                                                                                            \
     if (local_hist)                                                                                               \
       k_gkd_tiles<R, X, Y, Z, true><<<G.ntiles, kT, 0, st>>>(ps, phi, R(kick), R(drift), L, G, B.offsets,         \
-                                                             B.entries, B.counts, B.capacity, tab);               \
+                                                             B.entries, B.counts, B.capacity, tab, edge);         \
     else                                                                                                          \
       k_gkd_tiles<R, X, Y, Z, false><<<G.ntiles, kT, 0, st>>>(ps, phi, R(kick), R(drift), L, G, B.offsets,        \
-                                                              B.entries, B.counts, B.capacity, tab);              \
+                                                              B.entries, B.counts, B.capacity, tab, edge);        \
   } while (0)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
@@ -1248,7 +1313,8 @@ hipError_t batch_scan(const BatchDesc* d, long total, int n_members, hipStream_t
   template hipError_t deposit_tiles<R>(const PState<R>&, R*, double, const Lattice&, const TileGeom&,       \
                                        const Bins&, hipStream_t, int*, double);                              \
   template hipError_t gkd_tiles<R>(const PState<R>&, const PState<R>&, const R*, double, double,            \
-                                   const Lattice&, const TileGeom&, const Bins&, const StepTable&, hipStream_t);
+                                   const Lattice&, const TileGeom&, const Bins&, const StepTable&, hipStream_t, \
+                                   const float4*);
 PM_INST(float)
 PM_INST(double)
 #undef PM_INST

@@ -16,7 +16,7 @@ PM_OK = 0
 PM_FLAG_CIC_DECONVOLVE = 1
 PM_FLAG_FIX_F_COSMOLOGY = 2
 PM_FLAG_NO_GRAPH = 4
-PM_FLAG_SEPARATE_DEPOSIT = 8
+PM_FLAG_FUSED_DEPOSIT = 8
 PM_FLAG_DETERMINISTIC = 16
 PM_E_CORRUPT = 6
 

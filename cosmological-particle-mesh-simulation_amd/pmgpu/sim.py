@@ -21,7 +21,7 @@ from . import _lib
 class PMSim:
     def __init__(self, n_cells: int, n_parts: int, precision: int = 32, *,
                  cic_deconvolve: bool = False, fix_f_cosmology: bool = False, graph: bool = True,
-                 separate_deposit: bool = False, deterministic: bool = False,
+                 fused_deposit: bool = False, deterministic: bool = False,
                  omega_m0: float = 0.31, omega_lambda0: float = 0.69, omega_k0: float = 0.0,
                  h0: float = 0.68):
         _lib.init()
@@ -36,8 +36,8 @@ class PMSim:
             flags |= _lib.PM_FLAG_FIX_F_COSMOLOGY
         if not graph:
             flags |= _lib.PM_FLAG_NO_GRAPH
-        if separate_deposit:
-            flags |= _lib.PM_FLAG_SEPARATE_DEPOSIT
+        if fused_deposit:
+            flags |= _lib.PM_FLAG_FUSED_DEPOSIT
         if deterministic:
             flags |= _lib.PM_FLAG_DETERMINISTIC
         h = C.c_void_p()

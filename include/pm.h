@@ -41,10 +41,10 @@ extern "C" {
 #define PM_FLAG_FIX_F_COSMOLOGY 2u  /* f() gets [Omega_m, OL, OK] instead of [H0, OL, OK]     */
 #define PM_FLAG_NO_GRAPH 4u         /* pm_sim_run enqueues every kernel instead of replaying a
                                        captured graph of consecutive steps                     */
-#define PM_FLAG_SEPARATE_DEPOSIT 8u /* FP32: deposit into the density grid and run the R2C x
-                                       pass on it instead of the fused row-block kernel that
-                                       transforms the deposit in LDS (the default where the grid
-                                       size and tile shape allow it)                           */
+#define PM_FLAG_FUSED_DEPOSIT 8u    /* FP32, N = 256/384/512, 32x8x8 or 64x8x8 tiles: deposit
+                                       each row block of 8 x 8 tile rows into LDS and run the R2C
+                                       x pass from there (the density grid is never stored);
+                                       opt-in: slower than the separate passes at 512^3       */
 #define PM_FLAG_DETERMINISTIC 16u   /* deposit with fixed-point LDS sums whose rounding does not
                                        depend on the order of the particles: repeated runs from
                                        one state give bit-identical results                    */

@@ -46,7 +46,9 @@ def test_pure_host_entry_points():
     # 8^3: 2 * 64^3 counters in chunks of 4096, plus the spine's ticket counter)
     ntiles = (512 // 8) ** 3
     bins = 4 * (3 * 2 * ntiles + 1 + 2 * ntiles // 4096 + 1) + 4 * 8 * 256 ** 3
-    # + a fixed 1 MiB for the step table, graphs and health counters
-    assert b.value == 2 * (6 * 4 + 4) * 256 ** 3 + 2 * 512 ** 3 * 4 + 512 * 512 * 2 * 4 + bins + (1 << 20)
+    # + a fixed 1 MiB for the step table, graphs and health counters + the gather's x-margin
+    # array (a float4 per row and multiple of 32)
+    assert b.value == (2 * (6 * 4 + 4) * 256 ** 3 + 2 * 512 ** 3 * 4 + 512 * 512 * 2 * 4 + bins
+                       + (1 << 20) + 512 * 512 * 16 * 16)
     assert lib.pm_sim_footprint(512, 256, 16, ctypes.byref(b)) != 0
     assert lib.pm_supported_grid(512) == 1 and lib.pm_supported_grid(100) == 0

@@ -428,7 +428,7 @@ def test_fused_deposit_r2c_matches_separate_passes(n, npart, tiles):
     from pmgpu.sim import PMSim
     pos, vel = make_ics(n, npart, seed=41, box=float(n))
     kw = dict(omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
-    fused, sep = PMSim(n, npart, 32, **kw), PMSim(n, npart, 32, separate_deposit=True, **kw)
+    fused, sep = PMSim(n, npart, 32, fused_deposit=True, **kw), PMSim(n, npart, 32, **kw)
     ref = PMSim(n, npart, 64, **kw)
     for s in (fused, sep):
         s.set_tiles(*tiles)
@@ -464,10 +464,12 @@ def test_fused_deposit_r2c_matches_separate_passes(n, npart, tiles):
 
 
 def test_fused_deposit_step_matches_oracle():
+    from pmgpu.sim import PMSim
     n, npart = 256, 32
     pos, vel = make_ics(n, npart, seed=43, box=256.0)
-    sim = _sim(n, npart, 32)
-    assert sim.tiles() == (32, 8, 8)             # the FP32 default at this size runs fused
+    sim = PMSim(n, npart, 32, fused_deposit=True, omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0,
+                h0=0.7)
+    assert sim.tiles() == (32, 8, 8)             # the FP32 default tile shape at this size
     sim.set_state(pos, vel)
     x0, v0 = sim.get_state(np.float64)
     p = O.OracleParams(n_cells=n, n_parts=npart, omega_m0=0.3, omega_lambda0=0.7, h0=0.7)
