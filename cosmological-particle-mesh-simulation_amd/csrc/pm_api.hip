@@ -44,6 +44,8 @@ bool g_init = false;
 int g_device = 0;
 hipStream_t g_stream = nullptr;
 
+int bind_scratch(int device);
+
 int ensure_init() {
   if (g_init) return PM_OK;
   std::lock_guard<std::mutex> lk(g_mu);
@@ -53,6 +55,7 @@ int ensure_init() {
   if (ndev <= 0) return fail(PM_E_HIP, "no HIP device visible");
   PM_HIP(hipGetDevice(&g_device));
   PM_HIP(hipStreamCreate(&g_stream));
+  if (int rc = bind_scratch(g_device)) return rc;
   g_init = true;
   return PM_OK;
 }
@@ -135,6 +138,50 @@ int get_workspace(int n, bool need_phi, Workspace** out) {
   return PM_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// fixed device scratch of pm_potential_static: the complex128 half spectrum, the Nyquist plane
+// and the FFT tables of one N^3 grid, N <= kStaticMaxN (about 17 MiB, held by each device the
+// library binds to for the life of the process).  Host code never uses the array's name for
+// device memory: the address comes from hipGetSymbolAddress, taken once per device at set-up.
+// ------------------------------------------------------------------------------------------
+constexpr int kStaticMaxN = 128;
+constexpr size_t kStaticH = (size_t)kStaticMaxN * kStaticMaxN * (kStaticMaxN / 2) * sizeof(double2);
+constexpr size_t kStaticQ = (size_t)kStaticMaxN * kStaticMaxN * sizeof(double2);
+constexpr size_t kStaticTables = 4 * (size_t)kStaticMaxN * sizeof(double);  // tw (2N), sin2, wdec
+constexpr size_t kStaticBytes = kStaticH + kStaticQ + kStaticTables;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+void* g_scratch[kMaxDevices] = {};  // device address of `scratch` on each device
+
+// with `device` current: look up this device's copy of the scratch array
+int bind_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(PM_E_INVALID, "device %d out of range", device);
+  if (g_scratch[device]) return PM_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(PM_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+  g_scratch[device] = ptr;
+  return PM_OK;
+}
+
+// FFT tables of size n computed on the device (the same values get_tables uploads)
+__global__ void k_static_tables(int n, double* __restrict__ tw, double* __restrict__ s2, double* __restrict__ wd) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const double ang = 2.0 * M_PI * (double)m / (double)n;
+  tw[2 * m] = cos(ang);
+  tw[2 * m + 1] = -sin(ang);
+  const double sp = sin(M_PI * (double)m / (double)n);
+  s2[m] = sp * sp;
+  const int mm = m < n / 2 ? m : m - n;
+  const double x = M_PI * (double)mm / (double)n;
+  const double sinc = mm == 0 ? 1.0 : sin(x) / x;
+  wd[m] = 1.0 / (sinc * sinc * sinc * sinc);
+}
+
 int check_grid(int n) {
   if (!fft_size_supported(n))
     return fail(PM_E_INVALID,
@@ -170,6 +217,7 @@ PM_API int pm_init(int device) {
   PM_HIP(hipSetDevice(device));
   g_device = device;
   PM_HIP(hipStreamCreate(&g_stream));
+  if (int rc = bind_scratch(device)) return rc;
   g_init = true;
   return PM_OK;
 }
@@ -305,6 +353,43 @@ PM_API int pm_potential(const float* rho, const float* fgrid, int n, double a, d
   if (int rc = check_grid(n)) return rc;
   if (!rho || !fgrid || !phi || a == 0.0) return fail(PM_E_INVALID, "pm_potential: bad arguments");
   if (int rc = potential_into(rho, fgrid, n, a, omega_m0, phi)) return rc;
+  PM_HIP(hipStreamSynchronize(g_stream));
+  return PM_OK;
+}
+
+// potential.py:7-10 like pm_potential, allocating nothing: the workspace and the FFT tables are
+// carved out of this device's `scratch` array.  Calls that overlap on one device race.
+PM_API int pm_potential_static(const float* rho, const float* fgrid, int n, double a, double omega_m0, float* phi) {
+  PM_INIT();
+  if (int rc = check_grid(n)) return rc;
+  if (n > kStaticMaxN)
+    return fail(PM_E_UNSUPPORTED, "pm_potential_static: N_CELLS=%d exceeds its fixed scratch (N_CELLS <= %d)", n,
+                kStaticMaxN);
+  if (!rho || !fgrid || !phi || a == 0.0) return fail(PM_E_INVALID, "pm_potential_static: bad arguments");
+  unsigned char* base = static_cast<unsigned char*>(g_scratch[g_device]);
+  if (!base) return fail(PM_E_STATE, "pm_potential_static: no scratch bound to device %d", g_device);
+  const size_t nn = (size_t)n;
+  double2* H = reinterpret_cast<double2*>(base);
+  double2* Q = reinterpret_cast<double2*>(base + nn * nn * (nn / 2) * sizeof(double2));
+  double* tab = reinterpret_cast<double*>(base + nn * nn * (nn / 2) * sizeof(double2) + nn * nn * sizeof(double2));
+  FftTables t;
+  t.n = n;
+  t.prec = 8;
+  t.tw = tab;
+  t.sin2 = tab + 2 * nn;
+  t.wdec = tab + 3 * nn;
+  // every input the passes read is rewritten here: the tables, then the whole half spectrum
+  k_static_tables<<<(n + 127) / 128, 128, 0, g_stream>>>(n, (double*)t.tw, (double*)t.sin2, (double*)t.wdec);
+  PM_HIP(hipGetLastError());
+  GreenArgs g{};
+  g.coef = (-3 * omega_m0 / 8 / a) / ((double)n * n * n);
+  g.from_grid = 1;
+  g.fgrid = fgrid;
+  PM_HIP((fft_r2c_x<double, float>(rho, H, Q, n, t, g_stream)));
+  PM_HIP(fft_c2c_y<double>(H, Q, n, -1, t, g_stream));
+  PM_HIP(fft_z_green<double>(H, Q, n, g, t, g_stream));
+  PM_HIP(fft_c2c_y<double>(H, Q, n, +1, t, g_stream));
+  PM_HIP((fft_c2r_x<double, float>(H, Q, phi, n, t, g_stream)));
   PM_HIP(hipStreamSynchronize(g_stream));
   return PM_OK;
 }

@@ -50,6 +50,7 @@ SIGNATURES = {
     "pm_potential_k": (_i, [_vp, _vp, _i, _d, _d, _vp]),
     "pm_potential_real": (_i, [_vp, _i, _vp]),
     "pm_potential": (_i, [_vp, _vp, _i, _d, _d, _vp]),
+    "pm_potential_static": (_i, [_vp, _vp, _i, _d, _d, _vp]),
     "pm_cell_centers": (_i, [_vp, _i64, _i, _vp]),
     "pm_weights": (_i, [_vp, _vp, _i64, _vp]),
     "pm_sweep_one_direction": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _i, _d, _d, _d, _i]),

@@ -79,6 +79,14 @@ PM_API int pm_potential_real(const void* d_phi_k, int n_cells, float* d_phi);
 /* potential.py:7-10  potential(density, fgrid, a), fused (complex128 arithmetic).           */
 PM_API int pm_potential(const float* d_rho, const float* d_fgrid, int n_cells, double a,
                         double omega_m0, float* d_phi);
+/* No counterpart in the reference: pm_potential for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as pm_potential; its complex128 workspace and FFT tables live in a fixed
+ * device array of about 17 MiB that the library holds on each device it binds to (found at
+ * pm_init), so N_CELLS is limited to 128 (PM_E_UNSUPPORTED above that) and the call allocates
+ * nothing.  Calls that overlap on one device share that array and race.                    */
+PM_API int pm_potential_static(const float* d_rho, const float* d_fgrid, int n_cells, double a,
+                               double omega_m0, float* d_phi);
 /* integrate.py:20  cell_centers = floor(positions).astype(int64) % N_CELLS                  */
 PM_API int pm_cell_centers(const float* d_positions, int64_t n_p, int n_cells,
                            int64_t* d_cells);
