@@ -505,6 +505,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=None)
     ap.add_argument("--tiles", type=lambda t: tuple(int(x) for x in t.split(",")), default=None,
                     help="fixed tile shape TX,TY,TZ of the 256^3/512^3 runs instead of the autotuner's")
+    ap.add_argument("--no-late", action="store_true",
+                    help="skip the late-time window (a whole run to a = 1)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a known-size kernel for "
                          "calibrating FETCH_SIZE / WRITE_SIZE counters)")
@@ -590,7 +592,7 @@ def main():
     out["step_roofline"]["frac_of_copy_ceiling"] = (out["step_roofline"]["achieved_GBs"]
                                                     / out["copy_ceiling"]["GBs"])
     total = loop_steps(w["a_init"], w["a_end"], w["steps"])
-    if args.steps < total:
+    if args.steps < total and not args.no_late:
         log(f"late-time window: the last {args.steps} steps of the run")
         out["late_time"] = late_time(pos, vel, w, args.steps, tiles32)
     log("plain FP64 GPU path (reference-layout entry points)")

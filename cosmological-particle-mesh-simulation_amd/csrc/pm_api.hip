@@ -667,6 +667,11 @@ StepScalars step_scalars(const pm_sim* s, double a, double da) {
 template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool table = false) {
   const int n = s->L.n;
   const bool fused = sizeof(R) == 4 && fused_deposit(s);
+  // Traversal directions: the R2C pass (after the deposit) and the C2R pass (after the inverse
+  // y pass) walk every XCD's run of row blocks backwards, so each first reads what its producer,
+  // walking the same planes forward, wrote last and the 256 MB last-level cache still holds, and
+  // the forward y pass and the gather after them find the same.
+  const bool rev = !(s->flags & PM_FLAG_FORWARD_ORDER);
   if (!s->spec_valid && (!s->rho_valid || !s->bins_valid)) {
     if (table) return fail(PM_E_STATE, "a captured step needs a valid density");
     if (fused) {
@@ -693,7 +698,7 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
   cpx<R>* H = (cpx<R>*)(in_rho_slot ? rho_slot(s) : spec_slot(s));
   if (!in_rho_slot) {
     StageTimer t(s, ST_R2C);
-    PM_HIP((fft_r2c_x<R, R>((const R*)rho_slot(s), H, Q, n, *s->tab, s->stream)));
+    PM_HIP((fft_r2c_x<R, R>((const R*)rho_slot(s), H, Q, n, *s->tab, s->stream, rev)));
   }
   // the other slot is the y / z passes' work array (the density it may hold has been consumed)
   cpx<R>* work = reinterpret_cast<cpx<R>*>(in_rho_slot ? spec_slot(s) : rho_slot(s));
@@ -713,7 +718,7 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
   float4* edge = (s->edge && s->G.tx == 32) ? s->edge : nullptr;
   {
     StageTimer t(s, ST_C2R);
-    PM_HIP((fft_c2r_x<R, R>(H, Q, (R*)H, n, *s->tab, s->stream, edge)));
+    PM_HIP((fft_c2r_x<R, R>(H, Q, (R*)H, n, *s->tab, s->stream, edge, rev)));
   }
   if (in_rho_slot) s->gb ^= 1;  // the potential now sits in the spectrum slot
   s->phi_valid = true;

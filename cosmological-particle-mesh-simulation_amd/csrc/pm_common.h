@@ -94,6 +94,16 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
   const int xcd = b % kXcds, local = b / kXcds;
   return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
+// rev: every XCD walks its run from the end instead.  Kernels that consume the previous
+// kernel's output alternate directions, so each reads first what its producer (walking the same
+// runs the other way) wrote last, which is still in the 256 MB last-level cache.
+__device__ __forceinline__ int xcd_block(int b, int nblocks, int rev) {
+  const int per = nblocks / kXcds, rem = nblocks % kXcds;
+  const int xcd = b % kXcds, local = b / kXcds;
+  const int len = xcd < rem ? per + 1 : per;
+  const int start = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+  return start + (rev ? len - 1 - local : local);
+}
 
 #define PM_OK 0
 #define PM_E_INVALID 1      // bad argument (size, pointer, unsupported grid)

@@ -270,9 +270,12 @@ struct FftTables {
 
 bool fft_size_supported(int n);
 
-// Host launchers (defined in pm_fft.hip).  All work on stream `st`.
+// Host launchers (defined in pm_fft.hip).  All work on stream `st`.  rev (x passes): the row
+// blocks are dealt to the XCDs in contiguous runs, each walked from its end, so the pass first
+// reads what a producer walking the same runs forward wrote last; otherwise dispatch order.
 template <typename R, typename Tin>
-hipError_t fft_r2c_x(const Tin* rho, cpx<R>* H, cpx<R>* Q, int n, const FftTables& t, hipStream_t st);
+hipError_t fft_r2c_x(const Tin* rho, cpx<R>* H, cpx<R>* Q, int n, const FftTables& t, hipStream_t st,
+                     bool rev = false);
 // work (optional): an N^2 (N/2)-element array the step's y / z passes may use (at N >= 1024 the
 // forward y pass leaves the spectrum there in a page-tiled layout for the z pass, and the inverse
 // y pass brings it back); both y passes and the z pass of one transform get the same work array
@@ -287,6 +290,6 @@ hipError_t fft_z_green(cpx<R>* H, cpx<R>* Q, int n, const GreenArgs& g, const Ff
 template <typename R, typename Tout>
 // edge (FP32 only, N % 32 == 0; may be null): also write the x margins of 32-wide tiles, see c2r_body
 hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTables& t, hipStream_t st,
-                     float4* edge = nullptr);
+                     float4* edge = nullptr, bool rev = false);
 
 }  // namespace pm

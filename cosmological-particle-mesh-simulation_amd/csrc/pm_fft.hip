@@ -120,11 +120,15 @@ __device__ __forceinline__ void r2c_body(const Tin* __restrict__ rho, cpx<R>* __
 }
 
 template <typename R, typename Tin, int N, int RB>
+// rev: the row blocks are dealt to the XCDs in contiguous runs, each walked from its end (the
+// pass then reads first what a producer walking the same runs forward wrote last, which the
+// last-level cache still holds); otherwise in dispatch order
 __global__ __launch_bounds__(kThreads) void k_r2c_x(const Tin* __restrict__ rho, cpx<R>* __restrict__ H,
-                                                    cpx<R>* __restrict__ Q, const cpx<R>* __restrict__ twg) {
+                                                    cpx<R>* __restrict__ Q, const cpx<R>* __restrict__ twg,
+                                                    int rev) {
   __shared__ cpx<R> lds[(N / 2) * (RB + 1)];
   __shared__ cpx<R> tw[N];
-  r2c_body<R, Tin, N, RB>(rho, H, Q, twg, blockIdx.x, lds, tw);
+  r2c_body<R, Tin, N, RB>(rho, H, Q, twg, rev ? xcd_block(blockIdx.x, gridDim.x, 1) : (int)blockIdx.x, lds, tw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -230,11 +234,13 @@ __device__ __forceinline__ void c2r_body(const cpx<R>* H, const cpx<R>* __restri
 
 template <typename R, typename Tout, int N, int RB>
 __global__ __launch_bounds__(kThreads) void k_c2r_x(const cpx<R>* H, const cpx<R>* __restrict__ Q, Tout* out,
-                                                    const cpx<R>* __restrict__ twg, float4* __restrict__ edge) {
+                                                    const cpx<R>* __restrict__ twg, float4* __restrict__ edge,
+                                                    int rev) {
   __shared__ cpx<R> lds[(N / 2) * (RB + 1)];
   __shared__ cpx<R> tw[N];
   __shared__ cpx<R> qx[RB];
-  c2r_body<R, Tout, N, RB>(H, Q, out, twg, blockIdx.x, lds, tw, qx, edge);
+  c2r_body<R, Tout, N, RB>(H, Q, out, twg, rev ? xcd_block(blockIdx.x, gridDim.x, 1) : (int)blockIdx.x, lds, tw,
+                           qx, edge);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -749,16 +755,17 @@ template <typename R, int N> void z_segments(PencilSeg& a, PencilSeg& b) {
 }
 
 template <typename R, typename Tin, int N>
-hipError_t launch_r2c(const Tin* rho, cpx<R>* H, cpx<R>* Q, const FftTables& t, hipStream_t st) {
+hipError_t launch_r2c(const Tin* rho, cpx<R>* H, cpx<R>* Q, const FftTables& t, hipStream_t st, bool rev) {
   constexpr int RB = rows_per_block<R, N>();
   static_assert(((long)N * N) % RB == 0, "rows per block must divide N^2");
-  k_r2c_x<R, Tin, N, RB><<<(long)N * N / RB, kThreads, 0, st>>>(rho, H, Q, (const cpx<R>*)t.tw);
+  k_r2c_x<R, Tin, N, RB><<<(long)N * N / RB, kThreads, 0, st>>>(rho, H, Q, (const cpx<R>*)t.tw, rev ? 1 : 0);
   return hipGetLastError();
 }
 template <typename R, typename Tout, int N>
-hipError_t launch_c2r(cpx<R>* H, const cpx<R>* Q, Tout* out, const FftTables& t, hipStream_t st, float4* edge) {
+hipError_t launch_c2r(cpx<R>* H, const cpx<R>* Q, Tout* out, const FftTables& t, hipStream_t st, float4* edge,
+                      bool rev) {
   constexpr int RB = rows_per_block<R, N>();
-  k_c2r_x<R, Tout, N, RB><<<(long)N * N / RB, kThreads, 0, st>>>(H, Q, out, (const cpx<R>*)t.tw, edge);
+  k_c2r_x<R, Tout, N, RB><<<(long)N * N / RB, kThreads, 0, st>>>(H, Q, out, (const cpx<R>*)t.tw, edge, rev ? 1 : 0);
   return hipGetLastError();
 }
 // Page-tiled spectrum for N >= 1024 (FP32).  The plain layout puts consecutive z of a column
@@ -838,9 +845,9 @@ bool fft_size_supported(int n) {
 }
 
 template <typename R, typename Tin>
-hipError_t fft_r2c_x(const Tin* rho, cpx<R>* H, cpx<R>* Q, int n, const FftTables& t, hipStream_t st) {
+hipError_t fft_r2c_x(const Tin* rho, cpx<R>* H, cpx<R>* Q, int n, const FftTables& t, hipStream_t st, bool rev) {
   switch (n) {
-#define PM_CASE(NN) case NN: return launch_r2c<R, Tin, NN>(rho, H, Q, t, st);
+#define PM_CASE(NN) case NN: return launch_r2c<R, Tin, NN>(rho, H, Q, t, st, rev);
     PM_FOR_EACH_N(PM_CASE)
 #undef PM_CASE
   }
@@ -848,9 +855,9 @@ hipError_t fft_r2c_x(const Tin* rho, cpx<R>* H, cpx<R>* Q, int n, const FftTable
 }
 template <typename R, typename Tout>
 hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTables& t, hipStream_t st,
-                     float4* edge) {
+                     float4* edge, bool rev) {
   switch (n) {
-#define PM_CASE(NN) case NN: return launch_c2r<R, Tout, NN>(H, Q, out, t, st, edge);
+#define PM_CASE(NN) case NN: return launch_c2r<R, Tout, NN>(H, Q, out, t, st, edge, rev);
     PM_FOR_EACH_N(PM_CASE)
 #undef PM_CASE
   }
@@ -894,17 +901,17 @@ hipError_t fft_z_green(cpx<R>* H, cpx<R>* Q, int n, const GreenArgs& g, const Ff
 }
 
 template hipError_t fft_r2c_x<float, float>(const float*, cpx<float>*, cpx<float>*, int, const FftTables&,
-                                             hipStream_t);
+                                             hipStream_t, bool);
 template hipError_t fft_r2c_x<double, double>(const double*, cpx<double>*, cpx<double>*, int, const FftTables&,
-                                               hipStream_t);
+                                               hipStream_t, bool);
 template hipError_t fft_r2c_x<double, float>(const float*, cpx<double>*, cpx<double>*, int, const FftTables&,
-                                              hipStream_t);
+                                              hipStream_t, bool);
 template hipError_t fft_c2r_x<float, float>(cpx<float>*, const cpx<float>*, float*, int,
-                                                   const FftTables&, hipStream_t, float4*);
+                                                   const FftTables&, hipStream_t, float4*, bool);
 template hipError_t fft_c2r_x<double, double>(cpx<double>*, const cpx<double>*, double*, int,
-                                                   const FftTables&, hipStream_t, float4*);
+                                                   const FftTables&, hipStream_t, float4*, bool);
 template hipError_t fft_c2r_x<double, float>(cpx<double>*, const cpx<double>*, float*, int,
-                                                   const FftTables&, hipStream_t, float4*);
+                                                   const FftTables&, hipStream_t, float4*, bool);
 #define PM_INST(R)                                                                                   \
   template hipError_t fft_c2c_y<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t, cpx<R>*); \
   template hipError_t fft_c2c_z<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t);       \

@@ -18,6 +18,7 @@ PM_FLAG_FIX_F_COSMOLOGY = 2
 PM_FLAG_NO_GRAPH = 4
 PM_FLAG_FUSED_DEPOSIT = 8
 PM_FLAG_DETERMINISTIC = 16
+PM_FLAG_FORWARD_ORDER = 32
 PM_E_CORRUPT = 6
 
 # N stages timed by pm_sim_get_timing, in order

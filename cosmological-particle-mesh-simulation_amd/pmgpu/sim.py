@@ -22,6 +22,7 @@ class PMSim:
     def __init__(self, n_cells: int, n_parts: int, precision: int = 32, *,
                  cic_deconvolve: bool = False, fix_f_cosmology: bool = False, graph: bool = True,
                  fused_deposit: bool = False, deterministic: bool = False,
+                 forward_order: bool = False,
                  omega_m0: float = 0.31, omega_lambda0: float = 0.69, omega_k0: float = 0.0,
                  h0: float = 0.68):
         _lib.init()
@@ -40,6 +41,8 @@ class PMSim:
             flags |= _lib.PM_FLAG_FUSED_DEPOSIT
         if deterministic:
             flags |= _lib.PM_FLAG_DETERMINISTIC
+        if forward_order:
+            flags |= _lib.PM_FLAG_FORWARD_ORDER
         h = C.c_void_p()
         _lib.call("pm_sim_create", self.n_cells, self.n_parts, self.precision, flags, C.byref(h))
         self._h = h

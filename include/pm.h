@@ -48,6 +48,11 @@ extern "C" {
 #define PM_FLAG_DETERMINISTIC 16u   /* deposit with fixed-point LDS sums whose rounding does not
                                        depend on the order of the particles: repeated runs from
                                        one state give bit-identical results                    */
+#define PM_FLAG_FORWARD_ORDER 32u   /* every step kernel walks its blocks forward (by default
+                                       the R2C and C2R passes walk them backwards, so they and
+                                       the kernels after them first read what their producer
+                                       wrote last and the last-level cache still holds; the
+                                       results are the same)                                   */
 
 /* ---------------------------------------------------------------- library / device memory */
 PM_API int pm_init(int device);                 /* select device, create the library stream */
