@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import re
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -145,6 +146,11 @@ class PMServer:
     the ensemble's chunk runs on its stream, so the device does not idle between chunks.  A
     request whose start fails, or whose state turns non-finite, ends with ``error`` set; the
     others are served normally.
+
+    ``cancel(id)`` (from any thread, or from ``on_result``) withdraws a request: one still
+    waiting ends at once, one being started or stepped at the next chunk boundary; its memory
+    and ensemble slot are released and its result carries ``error == "cancelled"`` and the
+    steps it took.  A request already retiring is served as usual.
     """
 
     def __init__(self, hbm_budget: int | None = None, max_members: int = MAX_MEMBERS,
@@ -162,6 +168,18 @@ class PMServer:
         self._ensemble = ensemble or PMEnsemble
         self._prepare = prepare or _prepare
         self.stats = {}
+        self._cancel_lock = threading.Lock()
+        self._cancel_ids: set[int] = set()
+
+    def cancel(self, request_id: int) -> None:
+        """Withdraw request `request_id` (served requests and unknown ids are ignored)."""
+        with self._cancel_lock:
+            self._cancel_ids.add(int(request_id))
+
+    def _take_cancels(self) -> set[int]:
+        with self._cancel_lock:
+            ids, self._cancel_ids = self._cancel_ids, set()
+        return ids
 
     def serve(self, requests, realtime: bool = True, clock=time.perf_counter,
               on_result=None) -> list[SimResult]:
@@ -183,7 +201,8 @@ class PMServer:
         busy = admit_s = retire_s = 0.0
         chunks = 0
         peak_members = 0
-        failed = 0
+        failed = cancelled = 0
+        withdrawn: set[int] = set()   # cancelled while being started: dropped when they join
         ens = self._ensemble()
         pool = cf.ThreadPoolExecutor(max_workers=self.prepare_workers)
         starter = cf.ThreadPoolExecutor(max_workers=1)
@@ -205,6 +224,35 @@ class PMServer:
         def fail_result(req, admitted, exc):
             return SimResult(req.id, arrival(req), admitted, now(), 0, req.a_init,
                              np.zeros(0), np.zeros(0), error=f"{type(exc).__name__}: {exc}")
+
+        def cancel_result(req, admitted, steps=0, a=None):
+            nonlocal cancelled
+            cancelled += 1
+            finish(SimResult(req.id, arrival(req), admitted, now(), steps,
+                             req.a_init if a is None else a, np.zeros(0), np.zeros(0),
+                             error="cancelled"))
+
+        def apply_cancels():
+            nonlocal used
+            ids = self._take_cancels()
+            if not ids:
+                return
+            for r in [r for r in pending if r.id in ids]:
+                pending.remove(r)
+                cancel_result(r, now())
+            for r in [r for r in queue if r.id in ids]:
+                queue.remove(r)
+                fut = prepared.pop(r.id, None)
+                if fut is not None:
+                    fut.cancel()
+                cancel_result(r, now())
+            withdrawn.update(r.id for r, _, _, _ in starting if r.id in ids)
+            for x in [x for x in active if x.req.id in ids]:
+                ens.remove(x.sim)
+                x.sim.close()
+                used -= x.fp
+                active.remove(x)
+                cancel_result(x.req, x.admitted, x.req.loop_steps - x.remaining, x.a)
 
         def start_job(req, fut):
             ta = clock()
@@ -248,8 +296,18 @@ class PMServer:
                     sim, dt = fut.result()
                 except Exception as exc:  # noqa: BLE001 - reported in the result
                     used -= fp
-                    failed += 1
-                    finish(fail_result(r, t_adm, exc))
+                    if r.id in withdrawn:
+                        withdrawn.discard(r.id)
+                        cancel_result(r, t_adm)
+                    else:
+                        failed += 1
+                        finish(fail_result(r, t_adm, exc))
+                    continue
+                if r.id in withdrawn:
+                    withdrawn.discard(r.id)
+                    sim.close()
+                    used -= fp
+                    cancel_result(r, t_adm)
                     continue
                 try:
                     ens.add(sim)
@@ -264,6 +322,7 @@ class PMServer:
 
         try:
             while pending or queue or active or starting or retiring:
+                apply_cancels()
                 t = now()
                 while pending and (not realtime or pending[0].arrival <= t):
                     r = pending.popleft()
@@ -329,9 +388,10 @@ class PMServer:
             ens.close()
             pool.shutdown(wait=True)
         wall = now()
+        self._take_cancels()  # ids of requests that had already finished
         self.stats = {"wall_s": wall, "device_busy_s": busy, "admit_s": admit_s,
                       "retire_s": retire_s, "chunks": chunks, "peak_members": peak_members,
-                      "requests": len(results), "failed": failed}
+                      "requests": len(results), "failed": failed, "cancelled": cancelled}
         return sorted(results, key=lambda r: r.id)
 
 

@@ -64,3 +64,23 @@ def test_realtime_arrivals_are_admitted_after_they_arrive():
         assert r.finished >= r.admitted
     lat = latency_summary(res)
     assert 0 < lat["p50_s"] <= lat["p99_s"] <= lat["max_s"]
+
+
+def test_cancelling_an_active_request_frees_it_and_the_rest_are_unchanged():
+    # request 1 (300 loop steps) is cancelled when request 0 (10 steps) completes: it ends at
+    # the next chunk boundary, partly stepped, its memory released; request 2 still equals its
+    # run alone
+    from pmgpu.serve import PMServer, SimRequest, run_alone
+    reqs = [SimRequest(id=0, seed=900, box=60.0, n_cells=64, n_parts=32, steps=11),
+            SimRequest(id=1, seed=901, box=90.0, n_cells=96, n_parts=48, steps=301),
+            SimRequest(id=2, seed=902, box=80.0, n_cells=64, n_parts=32, steps=41)]
+    server = PMServer(chunk_steps=8, keep_state=True)
+    res = server.serve(reqs, realtime=False,
+                       on_result=lambda r: server.cancel(1) if r.id == 0 else None)
+    assert [r.error for r in res] == [None, "cancelled", None]
+    assert 0 < res[1].steps < reqs[1].loop_steps
+    assert server.stats["cancelled"] == 1
+    alone = run_alone(reqs[2], keep_state=True)
+    assert res[2].steps == alone.steps and res[2].a_final == alone.a_final
+    assert np.abs(periodic_diff(res[2].positions, alone.positions, 64)).max() < 1e-4
+    assert rel(res[2].velocities, alone.velocities) < 1e-5

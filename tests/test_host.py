@@ -179,6 +179,50 @@ def test_a_member_that_turns_non_finite_is_retired_with_an_error(fake_device):
     assert all(s.closed for s in fake_device)
 
 
+class CancellingEnsemble(FakeEnsemble):
+    """Asks the server to cancel request 1 during the first batched call."""
+    server = None
+
+    def run(self, a, da, n_steps):
+        if len(FakeEnsemble.log) == 0:
+            CancellingEnsemble.server.cancel(1)
+        return super().run(a, da, n_steps)
+
+
+def test_cancelled_requests_release_their_slot_and_report_their_steps(fake_device):
+    # 0 and 1 run together (budget 32 = two 256^3 requests); 2 waits for a slot; 3 is
+    # cancelled before it is ever admitted.  Request 1 is cancelled during the first chunk:
+    # it ends at that chunk boundary with the steps it took, its sim is closed, and 2 is
+    # admitted into its budget without waiting for 0 to finish.
+    reqs = [_req(0, 256, 200), _req(1, 256, 200), _req(2, 256, 60), _req(3, 64, 40)]
+    server = PMServer(hbm_budget=32, max_members=8, chunk_steps=16, ensemble=CancellingEnsemble)
+    CancellingEnsemble.server = server
+    server.cancel(3)
+    out = server.serve(reqs, realtime=False)
+    assert [r.id for r in out] == [0, 1, 2, 3]
+    assert out[1].error == "cancelled" and out[1].steps == 16
+    assert out[3].error == "cancelled" and out[3].steps == 0
+    assert out[0].error is None and out[0].steps == reqs[0].loop_steps
+    assert out[2].error is None and out[2].steps == reqs[2].loop_steps
+    assert FakeEnsemble.log[0][0] == [0, 1]
+    # 2 was admitted into the released budget while 0 was still running (its start then
+    # runs in the starter thread and it joins at a later chunk boundary)
+    assert out[2].admitted < out[0].finished
+    assert all(3 not in ids for ids, _ in FakeEnsemble.log)
+    assert server.stats["cancelled"] == 2 and server.stats["failed"] == 0
+    assert [s.req.id for s in fake_device] == [0, 1, 2]   # 3 was never started
+    assert all(s.closed for s in fake_device)
+
+
+def test_cancelling_a_finished_or_unknown_request_changes_nothing(fake_device):
+    reqs = [_req(0, 64, 40), _req(1, 64, 40)]
+    server = PMServer(hbm_budget=10, max_members=8, chunk_steps=16)
+    server.cancel(7)
+    out = server.serve(reqs, realtime=False, on_result=lambda r: server.cancel(r.id))
+    assert all(r.error is None and r.steps == reqs[0].loop_steps for r in out)
+    assert server.stats["cancelled"] == 0
+
+
 def test_request_larger_than_the_budget_is_rejected(fake_device):
     with pytest.raises(ValueError, match="HBM budget"):
         PMServer(hbm_budget=10).serve([_req(0, 384, 50)], realtime=False)
