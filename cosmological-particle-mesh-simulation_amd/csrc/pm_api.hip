@@ -554,12 +554,12 @@ bool fused_deposit(const pm_sim* s) {
 }
 
 // Health counters: enqueue their copy to the pinned host mirror on `st` before the caller's
-// synchronisation, then judge them after it.  A problem clears the counters and fails.
+// synchronisation, then check them after it.  A problem clears the counters and fails.
 int health_fetch(pm_sim* s, hipStream_t st) {
   PM_HIP(hipMemcpyAsync(s->h_health, s->B.health, sizeof(uint32_t) * PM_HEALTH_COUNT, hipMemcpyDeviceToHost, st));
   return PM_OK;
 }
-int health_verdict(pm_sim* s, hipStream_t st) {
+int health_report(pm_sim* s, hipStream_t st) {
   const uint32_t* h = s->h_health;
   if (!(h[PM_HEALTH_NONFINITE] | h[PM_HEALTH_BINS] | h[PM_HEALTH_OVERFLOW])) return PM_OK;
   const uint32_t nf = h[PM_HEALTH_NONFINITE], nb = h[PM_HEALTH_BINS], no = h[PM_HEALTH_OVERFLOW];
@@ -822,7 +822,7 @@ template <typename R> int sim_run_graph(pm_sim* s, double& a, double da, int n_s
     // the host buffers must outlive the asynchronous copies above
     if (int rc = health_fetch(s, s->stream)) return rc;
     PM_HIP(hipStreamSynchronize(s->stream));
-    if (int rc = health_verdict(s, s->stream)) return rc;
+    if (int rc = health_report(s, s->stream)) return rc;
     done += chunk;
   }
   // replays end where they started (even step counts); the direct remainder steps kept the flags
@@ -1143,7 +1143,7 @@ PM_API int pm_sim_set_state(pm_sim* s, const void* pos, const void* vel, int dty
   s->bins_valid = false;
   s->rho_valid = s->spec_valid = false;
   s->phi_valid = false;
-  if (int rc = health_verdict(s, s->stream)) {
+  if (int rc = health_report(s, s->stream)) {
     s->has_state = false;  // a state with non-finite values is not kept
     return rc;
   }
@@ -1239,7 +1239,7 @@ PM_API int pm_sim_step(pm_sim* s, double a, double da) {
   if (rc) return rc;
   if (int rc2 = health_fetch(s, s->stream)) return rc2;
   PM_HIP(hipStreamSynchronize(s->stream));
-  return health_verdict(s, s->stream);
+  return health_report(s, s->stream);
 }
 
 PM_API int pm_sim_run(pm_sim* s, double a, double da, int n_steps, double* a_out) {
@@ -1260,7 +1260,7 @@ PM_API int pm_sim_run(pm_sim* s, double a, double da, int n_steps, double* a_out
   }
   if (int rc = health_fetch(s, s->stream)) return rc;
   PM_HIP(hipStreamSynchronize(s->stream));
-  if (int rc = health_verdict(s, s->stream)) return rc;
+  if (int rc = health_report(s, s->stream)) return rc;
   if (a_out) *a_out = a;
   return PM_OK;
 }
@@ -1742,7 +1742,7 @@ PM_API int pm_ensemble_run(pm_ensemble* e, double* a, const double* da, int n_st
     s->spec_valid = false;
   }
   for (int i = 0; i < nm; ++i)
-    if (int rc = health_verdict(e->members[i], e->stream)) {
+    if (int rc = health_report(e->members[i], e->stream)) {
       g_err = "ensemble member " + std::to_string(i) + " (after " + std::to_string(done) + " steps): " + g_err;
       return rc;
     }

@@ -422,7 +422,7 @@ def test_autotune_keeps_state_and_results():
 def test_fused_deposit_r2c_matches_separate_passes(n, npart, tiles):
     # the row-block kernel that deposits density.py:17-47 into LDS and runs the x transform of
     # potential.py:19-21 from there, against the tile deposit + R2C pass (FP64 LDS sums): both
-    # FP32 steps are judged against the FP64 step from the same state (the FP32 FFT's own
+    # FP32 steps are compared with the FP64 step from the same state (the FP32 FFT's own
     # rounding, ~1e-6 in the potential, dominates either), then over ten graph-replayed steps
     # and with the density read back in the middle of a run
     from pmgpu.sim import PMSim
