@@ -119,10 +119,10 @@ __device__ __forceinline__ void r2c_body(const Tin* __restrict__ rho, cpx<R>* __
   }
 }
 
-template <typename R, typename Tin, int N, int RB>
 // rev: the row blocks are dealt to the XCDs in contiguous runs, each walked from its end (the
 // pass then reads first what a producer walking the same runs forward wrote last, which the
 // last-level cache still holds); otherwise in dispatch order
+template <typename R, typename Tin, int N, int RB>
 __global__ __launch_bounds__(kThreads) void k_r2c_x(const Tin* __restrict__ rho, cpx<R>* __restrict__ H,
                                                     cpx<R>* __restrict__ Q, const cpx<R>* __restrict__ twg,
                                                     int rev) {
