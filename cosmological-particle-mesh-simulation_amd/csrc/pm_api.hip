@@ -489,6 +489,9 @@ struct pm_sim {
   hipStream_t stream = nullptr;
   bool has_state = false, rho_valid = false, phi_valid = false, bins_valid = false;
   bool spec_valid = false;      // the density slot holds the x spectrum of the current density
+  // the forward y, z and inverse y passes in their alternative pencil shape (fft_pencil_cols;
+  // picked by the autotuner or set by the caller)
+  bool pencil_alt[3] = {false, false, false};
   uint32_t* h_health = nullptr; // pinned host copy of the health counters (B.health on the device)
   // graph replay of consecutive steps: per-step scalars in a device table read at *d_step
   double* d_table = nullptr;   // kTableSteps x {green coefficient, kick, drift}
@@ -704,15 +707,15 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
   cpx<R>* work = reinterpret_cast<cpx<R>*>(in_rho_slot ? spec_slot(s) : rho_slot(s));
   {
     StageTimer t(s, ST_Y_FWD);
-    PM_HIP(fft_c2c_y<R>(H, Q, n, -1, *s->tab, s->stream, work));
+    PM_HIP(fft_c2c_y<R>(H, Q, n, -1, *s->tab, s->stream, work, s->pencil_alt[0]));
   }
   {
     StageTimer t(s, ST_Z_GREEN);
-    PM_HIP(fft_z_green<R>(H, Q, n, g, *s->tab, s->stream, work));
+    PM_HIP(fft_z_green<R>(H, Q, n, g, *s->tab, s->stream, work, s->pencil_alt[1]));
   }
   {
     StageTimer t(s, ST_Y_INV);
-    PM_HIP(fft_c2c_y<R>(H, Q, n, +1, *s->tab, s->stream, work));
+    PM_HIP(fft_c2c_y<R>(H, Q, n, +1, *s->tab, s->stream, work, s->pencil_alt[2]));
   }
   // the gather's 32-wide tiles read their x margins from the edge array the C2R pass writes
   float4* edge = (s->edge && s->G.tx == 32) ? s->edge : nullptr;
@@ -896,6 +899,40 @@ template <typename R> int sim_autotune_impl(pm_sim* s, int reps, double* best_ms
       bg = g;
     }
   }
+  // pencil shapes: each y / z pass that has an alternative shape is timed in both on the
+  // spectrum slot (its values do not matter) and keeps the alternative only when it is faster
+  // by more than 2 %
+  for (int p = 0; p < 3 && rc == PM_OK; ++p) {
+    const int mode = p == 0 ? -1 : (p == 1 ? 0 : 1);
+    if (!fft_pencil_cols(n, s->prec, mode, true)) {
+      s->pencil_alt[p] = false;
+      continue;
+    }
+    float t[2] = {0.f, 0.f};
+    hipError_t e = hipSuccess;
+    for (int alt = 0; alt < 2 && e == hipSuccess; ++alt) {
+      for (int r = -1; r < reps && e == hipSuccess; ++r) {  // r = -1: warm-up
+        if (r == 0) e = hipEventRecord(e0, s->stream);
+        cpx<R>* H = (cpx<R>*)spec_slot(s);
+        if (!e) {
+          if (mode == 0) {
+            GreenArgs g{};
+            e = fft_z_green<R>(H, (cpx<R>*)s->Q, n, g, *s->tab, s->stream, nullptr, alt != 0);
+          } else {
+            e = fft_c2c_y<R>(H, (cpx<R>*)s->Q, n, mode, *s->tab, s->stream, nullptr, alt != 0);
+          }
+        }
+      }
+      if (!e) e = hipEventRecord(e1, s->stream);
+      if (!e) e = hipEventSynchronize(e1);
+      if (!e) e = hipEventElapsedTime(&t[alt], e0, e1);
+    }
+    if (e) {
+      rc = fail(PM_E_HIP, "autotune timing failed: %s", hipGetErrorString(e));
+      break;
+    }
+    s->pencil_alt[p] = t[1] < 0.98f * t[0];
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   s->G = bg;
@@ -918,6 +955,34 @@ PM_API int pm_sim_set_tiles(pm_sim* s, int tx, int ty, int tz) {
   if (s->ens && !(tx == 16 && ty == 16 && tz == 16) && (tx || ty || tz))
     return fail(PM_E_STATE, "ensemble members keep 16^3 tiles");
   return sim_use_tiles(s, make_tiles(s->L.n, s->prec, tx, ty, tz));
+}
+
+PM_API int pm_sim_set_pencils(pm_sim* s, int y_fwd, int z, int y_inv) {
+  if (!s) return fail(PM_E_INVALID, "null sim");
+  const int want[3] = {y_fwd, z, y_inv};
+  bool alt[3];
+  for (int p = 0; p < 3; ++p) {
+    const int mode = p == 0 ? -1 : (p == 1 ? 0 : 1);
+    const int w0 = fft_pencil_cols(s->L.n, s->prec, mode, false), w1 = fft_pencil_cols(s->L.n, s->prec, mode, true);
+    if (want[p] != 0 && want[p] != w0 && (w1 == 0 || want[p] != w1))
+      return fail(PM_E_UNSUPPORTED, "%d columns per block is not a shape of pass %d at N_CELLS=%d (%d%s%s)", want[p],
+                  p, s->L.n, w0, w1 ? " or " : "", w1 ? std::to_string(w1).c_str() : "");
+    alt[p] = want[p] != 0 && want[p] != w0;
+  }
+  PM_HIP(hipStreamSynchronize(s->stream));
+  if (alt[0] != s->pencil_alt[0] || alt[1] != s->pencil_alt[1] || alt[2] != s->pencil_alt[2]) {
+    for (int p = 0; p < 3; ++p) s->pencil_alt[p] = alt[p];
+    sim_drop_graph(s);
+  }
+  return PM_OK;
+}
+
+PM_API int pm_sim_get_pencils(pm_sim* s, int* y_fwd, int* z, int* y_inv) {
+  if (!s || !y_fwd || !z || !y_inv) return fail(PM_E_INVALID, "null argument");
+  int* out[3] = {y_fwd, z, y_inv};
+  for (int p = 0; p < 3; ++p)
+    *out[p] = fft_pencil_cols(s->L.n, s->prec, p == 0 ? -1 : (p == 1 ? 0 : 1), s->pencil_alt[p]);
+  return PM_OK;
 }
 
 PM_API int pm_sim_get_tiles(pm_sim* s, int* tx, int* ty, int* tz) {

@@ -281,12 +281,16 @@ hipError_t fft_r2c_x(const Tin* rho, cpx<R>* H, cpx<R>* Q, int n, const FftTable
 // y pass brings it back); both y passes and the z pass of one transform get the same work array
 template <typename R>
 hipError_t fft_c2c_y(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, hipStream_t st,
-                     cpx<R>* work = nullptr);
+                     cpx<R>* work = nullptr, bool alt = false);
 template <typename R>
 hipError_t fft_c2c_z(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, hipStream_t st);
 template <typename R>
 hipError_t fft_z_green(cpx<R>* H, cpx<R>* Q, int n, const GreenArgs& g, const FftTables& t, hipStream_t st,
-                       cpx<R>* work = nullptr);
+                       cpx<R>* work = nullptr, bool alt = false);
+// Columns per block of a pencil pass (mode -1: forward y, 0: z with the k-space operator, +1:
+// inverse y) in its default shape, or with alt in its alternative one (0: none; the y / z
+// launchers take `alt` to run it; results are those of the default shape)
+int fft_pencil_cols(int n, int prec, int mode, bool alt);
 template <typename R, typename Tout>
 // edge (FP32 only, N % 32 == 0; may be null): also write the x margins of 32-wide tiles, see c2r_body
 hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTables& t, hipStream_t st,

@@ -263,8 +263,8 @@ __device__ __forceinline__ R green_factor(int kx, int ky, int kz, int n, const G
 }
 
 // LDS of a pencil pass: L*(W+1) complex + N twiddles (+ 2N reals of k-space tables)
-template <typename R, int N> constexpr int pencil_lds() {
-  return (N * (pencil_w<R, N>() + 1) + N) * (int)sizeof(cpx<R>) + 2 * N * (int)sizeof(R);
+template <typename R, int N, int W = pencil_w<R, N>()> constexpr int pencil_lds() {
+  return (N * (W + 1) + N) * (int)sizeof(cpx<R>) + 2 * N * (int)sizeof(R);
 }
 
 // the k-space operator's scalar prefactor of this step
@@ -301,10 +301,9 @@ template <typename R> struct PencilAddr {
   long tail;  // elements of the array from base to its end
 };
 
-template <typename R, int N>
+template <typename R, int N, int W = pencil_w<R, N>()>
 __device__ __forceinline__ PencilAddr<R> pencil_addr(cpx<R>* H, cpx<R>* Q, const PencilSeg& s0, const PencilSeg& s1,
                                                      long blk) {
-  constexpr int W = pencil_w<R, N>();
   const bool first = blk < s0.nblk;
   const PencilSeg& s = first ? s0 : s1;
   if (!first) blk -= s0.nblk;
@@ -353,12 +352,11 @@ __device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, int voff,
 }
 
 // elements of one pencil per thread
-template <typename R, int N> constexpr int pencil_lpt() { return (N * pencil_w<R, N>() + kThreads - 1) / kThreads; }
+template <typename R, int N, int W = pencil_w<R, N>()> constexpr int pencil_lpt() { return (N * W + kThreads - 1) / kThreads; }
 
 // element i of this thread: (row n, column w) of the pencil; contiguous pencils walk along w
-template <typename R, int N>
+template <typename R, int N, int W = pencil_w<R, N>()>
 __device__ __forceinline__ bool pencil_elem(int contig, int i, int tid, int& n, int& w) {
-  constexpr int W = pencil_w<R, N>();
   const int e = i * kThreads + tid;
   if (contig) {
     n = e / W;
@@ -373,11 +371,10 @@ __device__ __forceinline__ bool pencil_elem(int contig, int i, int tid, int& n, 
 // global -> LDS, in chunks of CH loads per thread issued before any is stored.  In a
 // contiguous pencil thread tid always handles column tid % W, and element i sits (NT / W) * i
 // rows below its first.
-template <typename R, int N>
+template <typename R, int N, int W = pencil_w<R, N>()>
 __device__ __forceinline__ void pencil_load(const PencilAddr<R>& p, cpx<R>* lds, int tid) {
-  constexpr int W = pencil_w<R, N>();
   constexpr int SN = W + 1;
-  constexpr int LPT = pencil_lpt<R, N>();
+  constexpr int LPT = pencil_lpt<R, N, W>();
   constexpr int CH = sizeof(R) == 4 ? 32 : 8;
   if (p.contig) {
     const cpx<R>* src = p.base + (tid / W) * p.stride_n + (tid % W);
@@ -402,12 +399,12 @@ __device__ __forceinline__ void pencil_load(const PencilAddr<R>& p, cpx<R>* lds,
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         int n, w;
-        if (i0 + i < LPT && pencil_elem<R, N>(0, i0 + i, tid, n, w)) u[i] = p.base[n + w * p.stride_w];
+        if (i0 + i < LPT && pencil_elem<R, N, W>(0, i0 + i, tid, n, w)) u[i] = p.base[n + w * p.stride_w];
       }
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         int n, w;
-        if (i0 + i < LPT && pencil_elem<R, N>(0, i0 + i, tid, n, w)) lds[n * SN + w] = u[i];
+        if (i0 + i < LPT && pencil_elem<R, N, W>(0, i0 + i, tid, n, w)) lds[n * SN + w] = u[i];
       }
     }
   }
@@ -499,15 +496,14 @@ __device__ __forceinline__ void pencil_transform_reg(const PencilAddr<R>& p, con
 // as LDS Stockham passes there, the z pass with its 4 MiB column stride is not)
 template <typename R, int N, int MODE> constexpr bool reg_pass() { return reg_transform<R, N>() && (N <= 512 || MODE == 0); }
 
-template <typename R, int N, int MODE>
+template <typename R, int N, int MODE, int W = pencil_w<R, N>()>
 __device__ __forceinline__ void pencil_transform(const PencilAddr<R>& p, const GreenArgs& g, R coef, cpx<R>* lds,
                                                  const cpx<R>* tw, const R* s2, const R* wd, int tid) {
   using T = cpx<R>;
   constexpr int L = N;
-  constexpr int W = pencil_w<R, N>();
   constexpr int SN = W + 1;
   constexpr int NT = kThreads;
-  if constexpr (reg_pass<R, N, MODE>()) {
+  if constexpr (W == pencil_w<R, N>() && reg_pass<R, N, MODE>()) {
     pencil_transform_reg<R, N, MODE>(p, g, coef, lds, tw, s2, wd, tid);
   } else if constexpr (MODE == -1) {
     fft_passes<T, L, 1, -1, W, SN, NT, 1>(lds, tw, tid);
@@ -527,11 +523,10 @@ __device__ __forceinline__ void pencil_transform(const PencilAddr<R>& p, const G
 }
 
 // LDS -> global
-template <typename R, int N>
+template <typename R, int N, int W = pencil_w<R, N>()>
 __device__ __forceinline__ void pencil_store(const PencilAddr<R>& p, const cpx<R>* lds, int tid) {
-  constexpr int W = pencil_w<R, N>();
   constexpr int SN = W + 1;
-  constexpr int LPT = pencil_lpt<R, N>();
+  constexpr int LPT = pencil_lpt<R, N, W>();
   if (p.contig) {
     cpx<R>* dst = p.base + (tid / W) * p.stride_n + (tid % W);
     const cpx<R>* src = lds + (tid / W) * SN + (tid % W);
@@ -543,7 +538,7 @@ __device__ __forceinline__ void pencil_store(const PencilAddr<R>& p, const cpx<R
 #pragma unroll 4
     for (int i = 0; i < LPT; ++i) {
       int n, w;
-      if (pencil_elem<R, N>(0, i, tid, n, w)) p.base[n + w * p.stride_w] = lds[n * SN + w];
+      if (pencil_elem<R, N, W>(0, i, tid, n, w)) p.base[n + w * p.stride_w] = lds[n * SN + w];
     }
   }
 }
@@ -565,7 +560,7 @@ __device__ __forceinline__ void pencil_tables(const cpx<R>* __restrict__ twg, co
 // Hout / s0out (optional): the first segment's pencils are written to Hout with the layout s0out
 // instead of back in place (the passes at N >= 1024 move the spectrum between layouts).
 // TAB_LDS: the tables are copied to LDS (tw, s2, wd) or are the global tables themselves.
-template <typename R, int N, int MODE, bool TAB_LDS = true>
+template <typename R, int N, int MODE, bool TAB_LDS = true, int W = pencil_w<R, N>()>
 __device__ __forceinline__ void pencil_body(cpx<R>* H, cpx<R>* Q, const PencilSeg& s0, const PencilSeg& s1,
                                             const GreenArgs& g, const cpx<R>* __restrict__ twg,
                                             const R* __restrict__ s2g, const R* __restrict__ wdg, long blk,
@@ -574,25 +569,25 @@ __device__ __forceinline__ void pencil_body(cpx<R>* H, cpx<R>* Q, const PencilSe
   const int tid = threadIdx.x;
   if constexpr (TAB_LDS)
     pencil_tables<R, N, MODE>(twg, s2g, wdg, const_cast<cpx<R>*>(tw), const_cast<R*>(s2), const_cast<R*>(wd), tid);
-  const PencilAddr<R> p = pencil_addr<R, N>(H, Q, s0, s1, blk);
-  pencil_load<R, N>(p, lds, tid);
+  const PencilAddr<R> p = pencil_addr<R, N, W>(H, Q, s0, s1, blk);
+  pencil_load<R, N, W>(p, lds, tid);
   __syncthreads();
   const R coef = MODE == 0 ? R(g.table ? g.table[3 * (*g.step)] : g.coef) : R(0);
-  pencil_transform<R, N, MODE>(p, g, coef, lds, tw, s2, wd, tid);
-  if (Hout && blk < s0.nblk) pencil_store<R, N>(pencil_addr<R, N>(Hout, Q, *s0out, s1, blk), lds, tid);
-  else pencil_store<R, N>(p, lds, tid);
+  pencil_transform<R, N, MODE, W>(p, g, coef, lds, tw, s2, wd, tid);
+  if (Hout && blk < s0.nblk) pencil_store<R, N, W>(pencil_addr<R, N, W>(Hout, Q, *s0out, s1, blk), lds, tid);
+  else pencil_store<R, N, W>(p, lds, tid);
 }
 
-template <typename R, int N, int MODE>
+template <typename R, int N, int MODE, int W = pencil_w<R, N>()>
 __global__ __launch_bounds__(kThreads) void k_pencil(cpx<R>* H, cpx<R>* Q, PencilSeg s0, PencilSeg s1, GreenArgs g,
                                                      const cpx<R>* __restrict__ twg, const R* __restrict__ s2g,
                                                      const R* __restrict__ wdg, cpx<R>* Hout, PencilSeg s0out) {
-  __shared__ cpx<R> lds[N * (pencil_w<R, N>() + 1)];
+  __shared__ cpx<R> lds[N * (W + 1)];
   __shared__ cpx<R> tw[N];  // (the Stockham passes read the twiddles in every stage: kept in LDS)
   __shared__ R s2[MODE == 0 ? N : 1];
   __shared__ R wd[MODE == 0 ? N : 1];
-  pencil_body<R, N, MODE>(H, Q, s0, s1, g, twg, s2g, wdg, xcd_block(blockIdx.x, gridDim.x), lds, tw, s2, wd, Hout,
-                          &s0out);
+  pencil_body<R, N, MODE, true, W>(H, Q, s0, s1, g, twg, s2g, wdg, xcd_block(blockIdx.x, gridDim.x), lds, tw, s2, wd,
+                                   Hout, &s0out);
 }
 
 // Register pass: the four-step transform with the columns read from and written to global
@@ -720,8 +715,7 @@ __global__ __launch_bounds__(kThreads) void k_pencil_reg(cpx<R>* H, cpx<R>* Q, P
 // ------------------------------------------------------------------------------------------
 // host-side segment descriptions
 // ------------------------------------------------------------------------------------------
-template <typename R, int N> void y_segments(PencilSeg& a, PencilSeg& b) {
-  constexpr int W = pencil_w<R, N>();
+template <typename R, int N, int W = pencil_w<R, N>()> void y_segments(PencilSeg& a, PencilSeg& b) {
   constexpr long M = N / 2;
   a = {};
   a.ncol_blk = M / W;
@@ -735,8 +729,7 @@ template <typename R, int N> void y_segments(PencilSeg& a, PencilSeg& b) {
   b.stride_n = 1;
   b.contig = 0;
 }
-template <typename R, int N> void z_segments(PencilSeg& a, PencilSeg& b) {
-  constexpr int W = pencil_w<R, N>();
+template <typename R, int N, int W = pencil_w<R, N>()> void z_segments(PencilSeg& a, PencilSeg& b) {
   constexpr long M = N / 2;
   a = {};
   a.ncol_blk = M / W;
@@ -796,17 +789,40 @@ template <typename R, int N> PencilSeg z_tiled(PencilSeg a) {  // outer y, rows 
   return a;
 }
 
+// The alternative shape of a Stockham pencil pass: half its default columns per block (less LDS,
+// more blocks per CU), for N >= 64 in the plain layout; 0 where there is none (register passes,
+// the page-tiled passes, widths below 2).
+template <typename R, int N, int MODE> constexpr int pencil_alt_w() {
+  constexpr int W2 = pencil_w<R, N>() / 2;
+  if constexpr (reg_pass<R, N, MODE>() || tiled_spectrum<R, N>() || N < 64 || W2 < 2) return 0;
+  else return ((N / 2) % W2 == 0 && N % W2 == 0 && kThreads % W2 == 0) ? W2 : 0;
+}
+
 // work (optional, N^2 (N/2) elements): the tiled work array of the step's y / z passes
+// alt: run the pass with its alternative shape (pencil_alt_w; hipErrorInvalidValue if none)
 template <typename R, int N, int MODE>
 hipError_t launch_pencil(cpx<R>* H, cpx<R>* Q, bool zaxis, const GreenArgs& g, const FftTables& t,
-                         hipStream_t st, cpx<R>* work = nullptr) {
+                         hipStream_t st, cpx<R>* work = nullptr, bool alt = false) {
+  const cpx<R>* tw = (const cpx<R>*)t.tw;
+  const R* s2 = (const R*)t.sin2;
+  const R* wd = (const R*)t.wdec;
+  if (alt) {
+    constexpr int W2 = pencil_alt_w<R, N, MODE>();
+    if constexpr (W2 > 0) {
+      PencilSeg a2, b2;
+      if (zaxis) z_segments<R, N, W2>(a2, b2);
+      else y_segments<R, N, W2>(a2, b2);
+      k_pencil<R, N, MODE, W2><<<(unsigned)(a2.nblk + b2.nblk), kThreads, 0, st>>>(H, Q, a2, b2, g, tw, s2, wd,
+                                                                                   nullptr, a2);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   PencilSeg a, b;
   if (zaxis) z_segments<R, N>(a, b);
   else y_segments<R, N>(a, b);
   const unsigned nb = (unsigned)(a.nblk + b.nblk);
-  const cpx<R>* tw = (const cpx<R>*)t.tw;
-  const R* s2 = (const R*)t.sin2;
-  const R* wd = (const R*)t.wdec;
   if constexpr (tiled_spectrum<R, N>()) {
     if (work) {
       if (zaxis) {  // in place on the tiled array
@@ -832,6 +848,22 @@ hipError_t launch_pencil(cpx<R>* H, cpx<R>* Q, bool zaxis, const GreenArgs& g, c
 }
 
 }  // namespace
+
+namespace {
+template <typename R, int N> int pencil_cols(int mode, bool alt) {
+  const int w2 = mode < 0 ? pencil_alt_w<R, N, -1>() : mode == 0 ? pencil_alt_w<R, N, 0>() : pencil_alt_w<R, N, 1>();
+  return alt ? w2 : pencil_w<R, N>();
+}
+}  // namespace
+
+int fft_pencil_cols(int n, int prec, int mode, bool alt) {
+  switch (n) {
+#define PM_CASE(NN) case NN: return prec == 4 ? pencil_cols<float, NN>(mode, alt) : pencil_cols<double, NN>(mode, alt);
+    PM_FOR_EACH_N(PM_CASE)
+#undef PM_CASE
+  }
+  return 0;
+}
 
 bool fft_size_supported(int n) {
   switch (n) {
@@ -864,13 +896,14 @@ hipError_t fft_c2r_x(cpx<R>* H, const cpx<R>* Q, Tout* out, int n, const FftTabl
   return hipErrorInvalidValue;
 }
 template <typename R>
-hipError_t fft_c2c_y(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, hipStream_t st, cpx<R>* work) {
+hipError_t fft_c2c_y(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, hipStream_t st, cpx<R>* work,
+                     bool alt) {
   GreenArgs g{};
   switch (n) {
 #define PM_CASE(NN)                                                                   \
   case NN:                                                                            \
-    return sign < 0 ? launch_pencil<R, NN, -1>(H, Q, false, g, t, st, work)           \
-                    : launch_pencil<R, NN, 1>(H, Q, false, g, t, st, work);
+    return sign < 0 ? launch_pencil<R, NN, -1>(H, Q, false, g, t, st, work, alt)      \
+                    : launch_pencil<R, NN, 1>(H, Q, false, g, t, st, work, alt);
     PM_FOR_EACH_N(PM_CASE)
 #undef PM_CASE
   }
@@ -891,9 +924,9 @@ hipError_t fft_c2c_z(cpx<R>* H, cpx<R>* Q, int n, int sign, const FftTables& t, 
 }
 template <typename R>
 hipError_t fft_z_green(cpx<R>* H, cpx<R>* Q, int n, const GreenArgs& g, const FftTables& t, hipStream_t st,
-                       cpx<R>* work) {
+                       cpx<R>* work, bool alt) {
   switch (n) {
-#define PM_CASE(NN) case NN: return launch_pencil<R, NN, 0>(H, Q, true, g, t, st, work);
+#define PM_CASE(NN) case NN: return launch_pencil<R, NN, 0>(H, Q, true, g, t, st, work, alt);
     PM_FOR_EACH_N(PM_CASE)
 #undef PM_CASE
   }
@@ -913,10 +946,10 @@ template hipError_t fft_c2r_x<double, double>(cpx<double>*, const cpx<double>*, 
 template hipError_t fft_c2r_x<double, float>(cpx<double>*, const cpx<double>*, float*, int,
                                                    const FftTables&, hipStream_t, float4*, bool);
 #define PM_INST(R)                                                                                   \
-  template hipError_t fft_c2c_y<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t, cpx<R>*); \
+  template hipError_t fft_c2c_y<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t, cpx<R>*, bool); \
   template hipError_t fft_c2c_z<R>(cpx<R>*, cpx<R>*, int, int, const FftTables&, hipStream_t);       \
   template hipError_t fft_z_green<R>(cpx<R>*, cpx<R>*, int, const GreenArgs&, const FftTables&, hipStream_t,  \
-                                     cpx<R>*);
+                                     cpx<R>*, bool);
 PM_INST(float)
 PM_INST(double)
 #undef PM_INST

@@ -77,6 +77,8 @@ SIGNATURES = {
     "pm_sim_footprint": (_i, [_i, _i, _i, C.POINTER(_sz)]),
     "pm_sim_set_tiles": (_i, [_vp, _i, _i, _i]),
     "pm_sim_get_tiles": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "pm_sim_set_pencils": (_i, [_vp, _i, _i, _i]),
+    "pm_sim_get_pencils": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
     "pm_sim_autotune": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]),
     "pm_white_noise": (_i, [_vp, _i64, C.c_uint64]),
     "pm_sim_zeldovich": (_i, [_vp, C.POINTER(_d), _i64, _d, _d, C.c_uint64, _vp, _i]),

@@ -81,8 +81,18 @@ class PMSim:
         _lib.call("pm_sim_get_tiles", self._h, *[C.byref(x) for x in t])
         return tuple(x.value for x in t)
 
+    def set_pencils(self, y_fwd: int = 0, z: int = 0, y_inv: int = 0):
+        """Columns per block of the forward y, z and inverse y FFT passes (0: the default)."""
+        _lib.call("pm_sim_set_pencils", self._h, int(y_fwd), int(z), int(y_inv))
+
+    def pencils(self) -> tuple[int, int, int]:
+        t = [C.c_int() for _ in range(3)]
+        _lib.call("pm_sim_get_pencils", self._h, *[C.byref(x) for x in t])
+        return tuple(x.value for x in t)
+
     def autotune(self, reps: int = 3):
-        """Pick the fastest tile shape on the current particles; returns (shape, ms per rep)."""
+        """Pick the fastest tile shape on the current particles, then each y / z FFT pass's
+        faster pencil shape; returns (tile shape, ms per rep of the particle stages)."""
         t = [C.c_int() for _ in range(3)]
         ms = C.c_double()
         _lib.call("pm_sim_autotune", self._h, int(reps), *[C.byref(x) for x in t], C.byref(ms))

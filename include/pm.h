@@ -165,8 +165,15 @@ PM_API int pm_sim_set_tiles(pm_sim* sim, int tx, int ty, int tz);
 PM_API int pm_sim_get_tiles(pm_sim* sim, int* tx, int* ty, int* tz);
 /* Autotuner: times deposit + gather + binning on the sim's own particles for every usable tile
  * shape (reps timed repetitions each, the state is left unchanged) and keeps the fastest;
- * returns the shape and its time per repetition in ms.                                     */
+ * returns the shape and its time per repetition in ms.  It then times each y / z FFT pass that
+ * has an alternative pencil shape in both shapes and keeps the faster (see pm_sim_get_pencils). */
 PM_API int pm_sim_autotune(pm_sim* sim, int reps, int* tx, int* ty, int* tz, double* best_ms);
+/* Pencil shape of the forward y, the z (with the k-space operator) and the inverse y FFT passes:
+ * columns per block.  Each pass runs in its default shape or, where it has one (the LDS passes
+ * for N_CELLS >= 64), in an alternative with half the columns; 0 = the default.  Results do not
+ * depend on the shape.  No reference counterpart (pyFFTW plans its own, potential.py:20,27). */
+PM_API int pm_sim_set_pencils(pm_sim* sim, int y_fwd, int z, int y_inv);
+PM_API int pm_sim_get_pencils(pm_sim* sim, int* y_fwd, int* z, int* y_inv);
 
 /* ------------------------------------------------ the step's two ends: ICs and the spectrum */
 /* gaussian_random_field.py:32-50 draws the field's normal deviates from an unseeded parallel
