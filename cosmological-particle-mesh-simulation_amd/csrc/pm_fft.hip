@@ -54,7 +54,9 @@ __device__ __forceinline__ void r2c_body(const Tin* __restrict__ rho, cpx<R>* __
   for (int i = tid; i < N; i += NT) tw[i] = twg[i];
   const size_t row0 = (size_t)blk * RB;
   const cpx<Tin>* src = reinterpret_cast<const cpx<Tin>*>(rho) + row0 * M;
-  // the block's loads are issued in chunks of CH per thread before they are stored
+  // the block's loads are issued in chunks of CH per thread before they are stored (the bound
+  // test is compiled out when the rows fill whole chunks: a branch around each load would keep
+  // the chunk in scratch memory, as it did for the FP64 C2R pass)
   constexpr int LPT = (RB * M + NT - 1) / NT;
   constexpr int CH = sizeof(R) == 4 ? 32 : 8;
 #pragma unroll
@@ -63,12 +65,12 @@ __device__ __forceinline__ void r2c_body(const Tin* __restrict__ rho, cpx<R>* __
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int e = (i0 + i) * NT + tid;
-      if (i0 + i < LPT && e < RB * M) u[i] = src[e];
+      if (i0 + i < LPT && ((RB * M) % NT == 0 || e < RB * M)) u[i] = src[e];
     }
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int e = (i0 + i) * NT + tid;
-      if (i0 + i < LPT && e < RB * M) {
+      if (i0 + i < LPT && ((RB * M) % NT == 0 || e < RB * M)) {
         const int w = e / M, n = e - w * M;
         lds[n * SN + w] = cmake<T>(R(u[i].x), R(u[i].y));
       }
@@ -159,12 +161,12 @@ __device__ __forceinline__ void c2r_body(const cpx<R>* H, const cpx<R>* __restri
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int e = (i0 + i) * NT + tid;
-      if (i0 + i < LPT && e < RB * M) u[i] = src[e];
+      if (i0 + i < LPT && ((RB * M) % NT == 0 || e < RB * M)) u[i] = src[e];
     }
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int e = (i0 + i) * NT + tid;
-      if (i0 + i < LPT && e < RB * M) {
+      if (i0 + i < LPT && ((RB * M) % NT == 0 || e < RB * M)) {
         const int w = e / M, n = e - w * M;
         lds[n * SN + w] = u[i];
       }
@@ -365,7 +367,7 @@ __device__ __forceinline__ bool pencil_elem(int contig, int i, int tid, int& n, 
     w = e / N;
     n = e - w * N;
   }
-  return e < N * W;
+  return (N * W) % kThreads == 0 || e < N * W;  // (a constant when the pencil fills whole rounds)
 }
 
 // global -> LDS, in chunks of CH loads per thread issued before any is stored.  In a
