@@ -47,7 +47,8 @@ extern "C" {
                                        opt-in: slower than the separate passes at 512^3       */
 #define PM_FLAG_DETERMINISTIC 16u   /* deposit with fixed-point LDS sums whose rounding does not
                                        depend on the order of the particles: repeated runs from
-                                       one state give bit-identical results                    */
+                                       one state give bit-identical states (the P(k)
+                                       estimator's FP64 shell sums still round in any order)  */
 #define PM_FLAG_FORWARD_ORDER 32u   /* every step kernel walks its blocks forward (by default
                                        the R2C and C2R passes walk them backwards, so they and
                                        the kernels after them first read what their producer

@@ -8,7 +8,8 @@ not need the float64 oracle to run at that size:
   CIC gather share their weights and the central difference is odd);
 * bit-exact particle-to-cell assignment of the reference-layout entry point (integrate.py:20);
 * the potential of the FP32 and FP64 steps against the same solve done with ROCm's FFT library;
-* the whole 1000-step run: P(k) of FP32 against FP64 within 0.5 % for k < k_Nyquist/2.
+* the whole 1000-step run: P(k) of FP32 against FP64 within 0.5 % for k < k_Nyquist/2;
+* the whole run twice in the fixed-point deposit mode: bit-identical positions and velocities.
 
 Against the CPU port of the reference (oracle/pm_port: density.py / integrate.py loop for loop
 with the reference's float32 storage and complex128 FFTs):
@@ -194,6 +195,25 @@ def test_full_size_run_power_spectrum_fp32_vs_fp64(ics):
     print(f"P(k) FP32 vs FP64 after 1000 steps (a = {a:.4f}): max rel err {err:.2e}")
     assert abs(a - 1.0) < 1e-9
     assert err < 5e-3
+
+
+def test_full_size_deterministic_run_is_bitwise_repeatable(ics):
+    # the whole 1000-step run twice in the fixed-point deposit mode: identical positions and
+    # velocities (the default FP64 LDS deposit's sums depend on the order its atomics land in,
+    # as density.py:17's racy scatter does).  The P(k) estimator sums its shells with FP64
+    # atomics, outside the mode: it agrees to rounding of those sums.
+    from pmgpu.sim import PMSim
+    pos, vel = ics
+    out = []
+    for rep in range(2):
+        s = PMSim(N, NP, 32, deterministic=True, omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+        s.set_state(pos, vel)
+        s.run(A0, DA, 1000)
+        out.append((*s.get_state(np.float32), s.power_spectrum(BOX)[1]))
+        s.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_allclose(out[0][2], out[1][2], rtol=1e-13, atol=0)
 
 
 def test_full_size_step_matches_cpu_port(ics):
