@@ -520,7 +520,8 @@ TileGeom max_tiles(int n, int prec) {
 size_t bins_bytes(int n, int np_side, int prec) {
   const TileGeom g = max_tiles(n, prec);
   const size_t np = (size_t)np_side * np_side * np_side;
-  return 4 * (3 * 2 * (size_t)g.ntiles + 1 + (size_t)scan_blocks(g) + 1) + 4 * 8 * np;
+  // counts (+ the tile scatter's arrays), offsets, cursors, scan spine, entries
+  return 4 * (counts_words(n, g) + 2 * 2 * (size_t)g.ntiles + 1 + (size_t)scan_blocks(g) + 1) + 4 * 8 * np;
 }
 
 // per-sim allocations besides the grids, particles and bins: the step table of graph replays,
@@ -616,7 +617,9 @@ int sim_collect_timing(pm_sim* s) { return collect_timing(s); }
 // bin every particle into its tile entries from the current state
 template <typename R> int sim_bin_impl(pm_sim* s) {
   StageTimer t(s, ST_BIN);
-  PM_HIP(hipMemsetAsync(s->B.counts, 0, 2 * sizeof(uint32_t) * s->G.ntiles, s->stream));
+  // counts and the tile scatter's far counts (after a change of tile shape they may cover what
+  // the previous shape's arrays held)
+  PM_HIP(hipMemsetAsync(s->B.counts, 0, 4 * sizeof(uint32_t) * s->G.ntiles, s->stream));
   PM_HIP(bins_count<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream));
   PM_HIP(bins_scan(s->G, s->B, s->stream));
   PM_HIP(bins_scatter<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream));
@@ -740,8 +743,8 @@ template <typename R> int sim_step_impl(pm_sim* s, double a, double da, bool tab
   {
     // the gather counted the entries of the new positions: finish their binning
     StageTimer t(s, ST_BIN);
-    PM_HIP(bins_scan(s->G, s->B, s->stream));
-    PM_HIP(bins_scatter<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream));
+    PM_HIP(bins_scan(s->G, s->B, s->stream, true));
+    PM_HIP(bins_scatter<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream, true));
   }
   int* step = table ? s->d_step : nullptr;
   if (fused) return sim_deposit_spec_impl(s, step);
@@ -883,8 +886,8 @@ template <typename R> int sim_autotune_impl(pm_sim* s, int reps, double* best_ms
                          StepTable{}, s->stream);
         if (!e) s->cur ^= 1;
       }
-      if (!e) e = bins_scan(s->G, s->B, s->stream);
-      if (!e) e = bins_scatter<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream);
+      if (!e) e = bins_scan(s->G, s->B, s->stream, true);
+      if (!e) e = bins_scatter<R>(cur_state<R>(s), s->L, s->G, s->B, s->stream, true);
     }
     if (!e) e = hipEventRecord(e1, s->stream);
     if (!e) e = hipEventSynchronize(e1);
@@ -1056,20 +1059,22 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
   const size_t m = 2 * (size_t)gmax.ntiles;
   if (8 * (size_t)s->L.np >= (size_t)UINT32_MAX) return cleanup(fail(PM_E_INVALID, "too many particles (N_PARTS > 812)"));
   s->B.capacity = (uint32_t)(8 * (size_t)s->L.np);
-  if (hipMalloc(&s->B.counts, m * 4) != hipSuccess || hipMalloc(&s->B.offsets, (m + 1) * 4) != hipSuccess ||
+  const size_t cw = counts_words(n_cells, gmax);
+  if (hipMalloc(&s->B.counts, cw * 4) != hipSuccess || hipMalloc(&s->B.offsets, (m + 1) * 4) != hipSuccess ||
       hipMalloc(&s->B.cursors, m * 4) != hipSuccess || hipMalloc(&s->B.entries, 8 * (size_t)s->L.np * 4) != hipSuccess ||
       hipMalloc(&s->B.block_sums, ((size_t)scan_blocks(gmax) + 1) * 4) != hipSuccess ||
       hipMalloc(&s->B.health, sizeof(uint32_t) * PM_HEALTH_COUNT) != hipSuccess ||
       hipHostMalloc(&s->h_health, sizeof(uint32_t) * PM_HEALTH_COUNT, hipHostMallocDefault) != hipSuccess)
     return cleanup(fail(PM_E_NOMEM, "device allocation of the tile bins failed"));
   std::memset(s->h_health, 0, sizeof(uint32_t) * PM_HEALTH_COUNT);
+  s->B.tile_scatter = true;
   // every entry holds a valid slot from the start, counters start at zero
   s->B.done = s->B.block_sums + scan_blocks(gmax);
   if (hipMemset(s->B.entries, 0, 8 * (size_t)s->L.np * 4) != hipSuccess ||
       hipMemset(s->idb[0], 0, (size_t)s->L.np * 4) != hipSuccess ||
       hipMemset(s->idb[1], 0, (size_t)s->L.np * 4) != hipSuccess ||
       hipMemset(s->B.done, 0, 4) != hipSuccess ||
-      hipMemset(s->B.counts, 0, m * 4) != hipSuccess || hipMemset(s->B.cursors, 0, m * 4) != hipSuccess ||
+      hipMemset(s->B.counts, 0, cw * 4) != hipSuccess || hipMemset(s->B.cursors, 0, m * 4) != hipSuccess ||
       hipMemset(s->B.offsets, 0, (m + 1) * 4) != hipSuccess ||
       hipMemset(s->B.health, 0, sizeof(uint32_t) * PM_HEALTH_COUNT) != hipSuccess)
     return cleanup(fail(PM_E_HIP, "clearing the tile bins failed"));

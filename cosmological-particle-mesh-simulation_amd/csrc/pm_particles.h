@@ -87,7 +87,7 @@ struct TileGeom {
 // segments together fill [0, N_p)) and halo segment [offsets[ntiles+t], offsets[ntiles+t+1]);
 // at most 8 entries per particle.
 struct Bins {
-  uint32_t* counts = nullptr;      // 2*ntiles (own keys, then halo keys), zero between steps
+  uint32_t* counts = nullptr;      // 2*ntiles (own keys, then halo keys), zero between steps; + tile scatter arrays
   uint32_t* offsets = nullptr;     // 2*ntiles + 1
   uint32_t* cursors = nullptr;     // 2*ntiles
   uint32_t* entries = nullptr;     // 8*N_p
@@ -95,6 +95,11 @@ struct Bins {
   uint32_t* done = nullptr;        // ticket counter of the scan's last-block spine (zero between scans)
   uint32_t capacity = 0;           // entries allocated (8*N_p)
   uint32_t* health = nullptr;      // PM_HEALTH_COUNT counters of problems the kernels found
+  // Tile scatter after a gather (false: the particle-parallel scatter with device cursors).
+  // Its arrays follow the 2*ntiles counts of the current shape in the counts buffer (sized by
+  // counts_words): 2*ntiles far counts (zero between binnings, like the counts), 128 reserved
+  // range starts per tile, the previous binning's ntiles + 1 own offsets.
+  bool tile_scatter = false;
 };
 
 // Tile shapes (x, y, z cells) the tile kernels are instantiated for; the autotuner picks
@@ -105,6 +110,13 @@ bool tile_shape_supported(int tx, int ty, int tz);
 TileGeom make_tiles(int n, int prec_bytes, int tx = 0, int ty = 0, int tz = 0);
 std::vector<TileGeom> tile_candidates(int n, int prec_bytes);
 int scan_blocks(const TileGeom& g);
+// tiles of at least this many cells, 4 or more per axis, take the tile scatter after a gather
+constexpr int kTileScatterCells = 2048;
+bool tile_scatter_ok(const TileGeom& g);
+// words of the counts buffer on an n^3 grid whose tile shape with most tiles is gmax: the counts
+// and far counts (4 per tile of any shape) and the tile scatter's 133 words per tile (+ 1) for
+// every shape it runs on
+size_t counts_words(int n, const TileGeom& gmax);
 
 // Resident particle state: displacements s and velocities v (3 x N_p each, component-major)
 // and the id (lattice slot) of the particle at each storage index.  A sim holds two of them;
@@ -127,11 +139,15 @@ hipError_t sim_get_state(const PState<R>& p, Tout* pos, Tout* vel, const Lattice
 hipError_t ids_identity(uint32_t* id, const Lattice& L, hipStream_t st);
 
 // ---- tile binning, owner-computes deposit, tile-staged gather + kick + drift ------------------
+// Binning: count (from positions) or a gather's counts, then scan and scatter.  after_gather:
+// the counts came from gkd_tiles, whose tile scatter buffers (when B has them and the shape
+// allows, tile_scatter_ok) the scan and the scatter then use.
 template <typename R>
 hipError_t bins_count(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
-hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st);
+hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st, bool after_gather = false);
 template <typename R>
-hipError_t bins_scatter(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st);
+hipError_t bins_scatter(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st,
+                        bool after_gather = false);
 // step (optional): a table-driven step's counter, advanced by the deposit as the step's last kernel.
 // det_fix > 0: order-independent fixed-point sums with det_fix units per unit mass
 // (deterministic_fix gives the one for a total mass).

@@ -96,6 +96,50 @@ __device__ __forceinline__ bool target_active(const int up[3], int o) {
   return ((o & 1) <= up[0]) && (((o >> 1) & 1) <= up[1]) && ((o >> 2) <= up[2]);
 }
 
+// Neighbourhood bins of a source tile: the 4^3 tiles at offsets -1..2 per axis, x {own, halo}
+// keys, 128 bins (bin = 2 * ((rz * 4 + ry) * 4 + rx) + kind).  A particle whose new home tile lies
+// at offset -1..1 has every corner target inside the neighbourhood; near_home_bin gives the bin
+// of its own key (false: moved further) and corner_bin that of corner offset o.
+__device__ __forceinline__ bool near_home_bin(const int tn[3], const int tc[3], const TileGeom& G, int& h0) {
+  const int ntc[3] = {G.ntx, G.nty, G.ntz};
+  int rb[3];
+  bool near = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int dlt = tn[a] - tc[a];
+    if (dlt > 2) dlt -= ntc[a];
+    if (dlt < -1) dlt += ntc[a];
+    rb[a] = dlt + 1;
+    near = near && dlt >= -1 && dlt <= 1;
+  }
+  h0 = 2 * ((rb[2] * 4 + rb[1]) * 4 + rb[0]);
+  return near;
+}
+__device__ __forceinline__ int corner_bin(int h0, int o) {
+  return o == 0 ? h0 : h0 + 2 * (((o >> 2) * 4 + ((o >> 1) & 1)) * 4 + (o & 1)) + 1;
+}
+// key of neighbourhood bin `bin` of the tile at (tix, tiy, tiz)
+__device__ __forceinline__ uint32_t bin_key(int tix, int tiy, int tiz, int bin, const TileGeom& G) {
+  const int cell = bin >> 1, kind = bin & 1;
+  const int rx = cell & 3, ry = (cell >> 2) & 3, rz = cell >> 4;
+  const int x = fold(tix + rx - 1, G.ntx), y = fold(tiy + ry - 1, G.nty), z = fold(tiz + rz - 1, G.ntz);
+  return (uint32_t)((z * G.nty + y) * G.ntx + x) + (kind ? (uint32_t)G.ntiles : 0u);
+}
+constexpr int kBins = 128;  // neighbourhood bins per source tile
+// Tile scatter arrays, laid out in the counts buffer after the 2*ntiles counts of the current
+// tile shape: 2*ntiles far counts (particles that left their source tile's neighbourhood), 128
+// reserved range starts per tile (kNone: empty bin), and the own offsets of the binning the
+// gather read (ntiles + 1; the scan keeps them before it overwrites them): the source range of
+// tile t, where the gather wrote its particles, is [old[t], old[t+1]).
+__host__ __device__ __forceinline__ size_t ts_far_at(long ntiles) { return 2 * (size_t)ntiles; }
+__host__ __device__ __forceinline__ size_t ts_base_at(long ntiles) { return 4 * (size_t)ntiles; }
+__host__ __device__ __forceinline__ size_t ts_old_at(long ntiles) { return (4 + (size_t)kBins) * ntiles; }
+__device__ __forceinline__ uint32_t* ts_base(uint32_t* counts, const TileGeom& G) {
+  return counts + ts_base_at(G.ntiles);
+}
+constexpr int kScatterT = 64;  // threads per block of the tile scatter (one wavefront per tile)
+constexpr int kScatterE = 4;   // particles per thread and batch in the tile scatter
+
 // Adds 1 to ctr[key] for every active lane with one atomic per distinct key in the wavefront
 // and returns each lane's rank among all additions to its key (old value + lane prefix).
 // Must be reached by every lane of the wavefront.
@@ -270,8 +314,105 @@ __global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, con
   bin_scatter_body<R, TX, TY, TZ>(s, id, L, G, offsets, cursors, entries, capacity, blockIdx.x, health);
 }
 
+// Scatter after a gather, one workgroup per source tile: the particles the gather of tile t
+// wrote (storage [srange[2t], srange[2t+1]) of its output) are placed through LDS cursors.  The
+// gather reserved, with the one counter atomic per (tile, neighbourhood bin) it issues anyway, a
+// range of every key's count space for the tile's particles (wbase; kNone for empty bins); the
+// cursor of a bin starts at offsets[key] + wbase, so the places cost LDS atomics, not device
+// atomics.  Particles that moved beyond the neighbourhood were counted in `far`; the scan put
+// them after the reserved ranges of each key, and they take a device cursor atomic.  The bins and
+// the neighbourhood test are recomputed from the stored positions, bit for bit those of the
+// gather.
+template <typename R, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict__ s, const uint32_t* __restrict__ id,
+                                                            Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
+                                                            uint32_t* __restrict__ cursors,
+                                                            uint32_t* __restrict__ entries, uint32_t capacity,
+                                                            uint32_t* __restrict__ health,
+                                                            uint32_t* __restrict__ counts) {
+  constexpr int NT = kScatterT, E = kScatterE, BPT = kBins / NT;
+  static_assert(kBins % NT == 0, "whole bins per thread");
+  __shared__ uint32_t cur[kBins], lim[kBins];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const long np = L.np;
+  const uint32_t* wbase = ts_base(counts, G);
+  const uint32_t* old = counts + ts_old_at(G.ntiles);
+  const uint32_t end = min(min(old[tile + 1], capacity), (uint32_t)np), beg = min(old[tile], end);
+  if (beg == end) return;  // block-uniform
+  const int tix = tile % G.ntx, tiy = (tile / G.ntx) % G.nty, tiz = tile / (G.ntx * G.nty);
+  const int tc[3] = {tix, tiy, tiz};
+  uint32_t w[BPT];
+#pragma unroll
+  for (int k = 0; k < BPT; ++k) w[k] = wbase[(size_t)tile * kBins + k * NT + threadIdx.x];
+  R sv[E][3];
+  uint32_t pid[E];
+  auto load = [&](uint32_t base) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const uint32_t e = base + u * NT + threadIdx.x;
+      const bool ok = e < end;
+      pid[u] = ok ? id[e] : 0u;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) sv[u][a] = ok ? s[a * np + e] : R(0);
+    }
+  };
+  load(beg);
+#pragma unroll
+  for (int k = 0; k < BPT; ++k) {
+    const int b = k * NT + threadIdx.x;
+    uint32_t c = 0u, l = 0u;
+    if (w[k] != kNone) {
+      const uint32_t key = bin_key(tix, tiy, tiz, b, G);
+      c = offsets[key] + w[k];
+      l = offsets[key + 1];
+    }
+    cur[b] = c;
+    lim[b] = l;
+  }
+  __syncthreads();
+  uint32_t lost = 0, nonfinite = 0;
+  for (uint32_t base = beg; base < end; base += NT * E) {
+    if (base != beg) load(base);
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const uint32_t e = base + u * NT + threadIdx.x;
+      if (e >= end) continue;
+      nonfinite += (__builtin_isfinite(sv[u][0]) && __builtin_isfinite(sv[u][1]) && __builtin_isfinite(sv[u][2])) ? 0u : 1u;
+      int c[3], tn[3], up[3];
+      R d[3], q[3];
+      cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], pid[u], L, c, d, q);
+      tile_of<TX, TY, TZ>(c, G, tn, up);
+      int h0;
+      if (near_home_bin(tn, tc, G, h0)) {
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          if (!target_active(up, o)) continue;
+          const int b = corner_bin(h0, o);
+          const uint32_t pos = atomicAdd(&cur[b], 1u);
+          if (pos < lim[b] && pos < capacity) entries[pos] = e;
+          else ++lost;
+        }
+      } else {
+#pragma unroll 1
+        for (int o = 0; o < 8; ++o) {
+          if (!target_active(up, o)) continue;
+          const uint32_t key = target_key(tn, o, G);
+          const uint32_t pos = offsets[key] + atomicAdd(&cursors[key], 1u);  // after the reserved ranges
+          if (pos < offsets[key + 1] && pos < capacity) entries[pos] = e;
+          else ++lost;
+        }
+      }
+    }
+  }
+  if (nonfinite && health) atomicAdd(&health[PM_HEALTH_NONFINITE], nonfinite);
+  if (lost && health) atomicAdd(&health[PM_HEALTH_OVERFLOW], lost);
+}
+
 // ------------------------------------------------------------------------------------------
-// exclusive scan of the 2*ntiles counts -> offsets (+ total at the end); clears the counters
+// exclusive scan of the 2*ntiles counts -> offsets (+ total at the end); clears the counters.
+// With `far` (after a gather whose tile scatter follows) a key's count is counts + far: the
+// ranges the gather reserved come first, and the cursor of the particles counted in `far`
+// starts after them.
 // ------------------------------------------------------------------------------------------
 // 4 counts per thread: 1024-count chunks, 128 blocks over the 2 x 65536 counts of a 512^3 grid
 // in 32x8x8 tiles (16 per thread left three quarters of the CUs idle: bin stage 183 -> 175 us;
@@ -305,12 +446,13 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 }
 
 __device__ __forceinline__ void scan_reduce_body(const uint32_t* __restrict__ counts, long m,
-                                                 uint32_t* __restrict__ bsum, long blk, uint32_t* sh) {
+                                                 uint32_t* __restrict__ bsum, long blk, uint32_t* sh,
+                                                 const uint32_t* __restrict__ far) {
   const long base = blk * (long)kScanChunk;
   uint32_t v = 0;
   for (int i = 0; i < kScanPer; ++i) {
     const long j = base + (long)i * kT + threadIdx.x;
-    if (j < m) v += counts[j];
+    if (j < m) v += counts[j] + (far ? far[j] : 0u);
   }
   uint32_t total;
   block_exclusive_scan(v, sh, total);
@@ -334,8 +476,9 @@ __device__ __forceinline__ void scan_spine_body(uint32_t* __restrict__ bsum, int
 // sums in place and resets the ticket counter for the next scan.  No block waits on another.
 __device__ __forceinline__ void scan_reduce_spine_body(const uint32_t* __restrict__ counts, long m,
                                                        uint32_t* __restrict__ bsum, int nb, uint32_t* done,
-                                                       long blk, uint32_t* sh, uint32_t* last) {
-  scan_reduce_body(counts, m, bsum, blk, sh);
+                                                       long blk, uint32_t* sh, uint32_t* last,
+                                                       const uint32_t* __restrict__ far = nullptr) {
+  scan_reduce_body(counts, m, bsum, blk, sh, far);
   if (threadIdx.x == 0) {
     __threadfence();
     *last = atomicAdd(done, 1u) == (uint32_t)(nb - 1) ? 1u : 0u;
@@ -349,22 +492,26 @@ __device__ __forceinline__ void scan_reduce_spine_body(const uint32_t* __restric
 }
 
 __global__ __launch_bounds__(kT) void k_scan_reduce_spine(const uint32_t* __restrict__ counts, long m,
-                                                          uint32_t* __restrict__ bsum, int nb, uint32_t* done) {
+                                                          uint32_t* __restrict__ bsum, int nb, uint32_t* done,
+                                                          const uint32_t* __restrict__ far) {
   __shared__ uint32_t sh[kT];
   __shared__ uint32_t last;
-  scan_reduce_spine_body(counts, m, bsum, nb, done, blockIdx.x, sh, &last);
+  scan_reduce_spine_body(counts, m, bsum, nb, done, blockIdx.x, sh, &last, far);
 }
 
 __device__ __forceinline__ void scan_apply_body(uint32_t* __restrict__ counts, long m,
                                                 const uint32_t* __restrict__ bsum, uint32_t* __restrict__ offsets,
-                                                uint32_t* __restrict__ cursors, long blk, uint32_t* sh) {
+                                                uint32_t* __restrict__ cursors, long blk, uint32_t* sh,
+                                                uint32_t* __restrict__ far = nullptr,
+                                                uint32_t* __restrict__ old = nullptr) {
   const long base = blk * (long)kScanChunk + (long)threadIdx.x * kScanPer;
-  uint32_t v[kScanPer];
+  uint32_t v[kScanPer], near[kScanPer];
   uint32_t sum = 0;
 #pragma unroll
   for (int i = 0; i < kScanPer; ++i) {
     const long j = base + i;
-    v[i] = j < m ? counts[j] : 0u;
+    near[i] = j < m ? counts[j] : 0u;
+    v[i] = near[i] + (far && j < m ? far[j] : 0u);
     sum += v[i];
   }
   uint32_t total;
@@ -373,9 +520,11 @@ __device__ __forceinline__ void scan_apply_body(uint32_t* __restrict__ counts, l
   for (int i = 0; i < kScanPer; ++i) {
     const long j = base + i;
     if (j < m) {
+      if (old && j <= m / 2) old[j] = offsets[j];  // own offsets of the binning being replaced
       offsets[j] = run;
       counts[j] = 0u;
-      cursors[j] = 0u;
+      cursors[j] = far ? near[i] : 0u;
+      if (far) far[j] = 0u;
     }
     run += v[i];
     if (j == m - 1) offsets[m] = run;
@@ -384,9 +533,10 @@ __device__ __forceinline__ void scan_apply_body(uint32_t* __restrict__ counts, l
 
 __global__ __launch_bounds__(kT) void k_scan_apply(uint32_t* __restrict__ counts, long m,
                                                    const uint32_t* __restrict__ bsum, uint32_t* __restrict__ offsets,
-                                                   uint32_t* __restrict__ cursors) {
+                                                   uint32_t* __restrict__ cursors, uint32_t* __restrict__ far,
+                                                   uint32_t* __restrict__ old) {
   __shared__ uint32_t sh[kT];
-  scan_apply_body(counts, m, bsum, offsets, cursors, blockIdx.x, sh);
+  scan_apply_body(counts, m, bsum, offsets, cursors, blockIdx.x, sh, far, old);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -735,7 +885,7 @@ template <typename R> struct GkdState {
 // and sl0 the thread's first batch of entries (fetched by the caller before the tile's
 // potential box, so the two dependent load chains overlap).  Own entry e's particle is
 // written to index e of the output state.
-template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
+template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST, bool TS>
 __device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restrict__ phi, R kick, R drift,
                                          const Lattice& L, const TileGeom& G,
                                          const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
@@ -748,7 +898,9 @@ __device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restr
   constexpr int ZIT = (BZ + NW - 1) / NW;           // planes per wavefront
   constexpr int E = kGkdE;
   // hist: new-tile counts, offset (-1..2)^3 from this tile x {own, halo}
+  static_assert(!TS || LOCAL_HIST, "the tile scatter needs the neighbourhood histogram");
   if (beg == end) return;  // empty tile: the whole block leaves together
+  const int wv_s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index in the block
   const int tix = tile % G.ntx, tiy = (tile / G.ntx) % G.nty, tiz = tile / (G.ntx * G.nty);
   const int o0[3] = {tix * TX, tiy * TY, tiz * TZ};
   const int n = L.n;
@@ -757,7 +909,7 @@ __device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restr
   // LOCAL_HIST: counts of the new tiles go through an LDS histogram of the 4x4x4 tile
   // neighbourhood, which needs 4 distinct tiles per axis to name it uniquely (the launcher
   // checks); otherwise each wavefront aggregates its own atomics
-  if (LOCAL_HIST && threadIdx.x < 128) hist[threadIdx.x] = 0u;
+  if (LOCAL_HIST && threadIdx.x < kBins) hist[threadIdx.x] = 0u;
   // The first batch of entries is fetched before the potential box, the batch's particle
   // state together with the box, so the two dependent load chains overlap.
   uint32_t sl[E], pid[E];
@@ -944,33 +1096,22 @@ __device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restr
         ps.id_out[eo] = pid[u];
         tile_of<TX, TY, TZ>(cn, G, tn, up);
         if (LOCAL_HIST) {
-          // new home tile relative to this one, folded into [-1, 2] per axis; when it lies in
-          // [-1, 1] every corner target (home + 0/1 per axis) is inside the 4^3 neighbourhood
           const int tc[3] = {tix, tiy, tiz};
-          const int ntc[3] = {G.ntx, G.nty, G.ntz};
-          int rb[3];
-          bool near = true;
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            int dlt = tn[a] - tc[a];
-            if (dlt > 2) dlt -= ntc[a];
-            if (dlt < -1) dlt += ntc[a];
-            rb[a] = dlt + 1;
-            near = near && dlt >= -1 && dlt <= 1;
-          }
-          if (near) {
-            const int h0 = 2 * ((rb[2] * 4 + rb[1]) * 4 + rb[0]);
+          int h0;
+          if (near_home_bin(tn, tc, G, h0)) {
             atomicAdd(&hist[h0], 1u);
             if (up[0] | up[1] | up[2]) {
 #pragma unroll
               for (int o = 1; o < 8; ++o)
-                if (target_active(up, o))
-                  atomicAdd(&hist[h0 + 2 * (((o >> 2) * 4 + ((o >> 1) & 1)) * 4 + (o & 1)) + 1], 1u);
+                if (target_active(up, o)) atomicAdd(&hist[corner_bin(h0, o)], 1u);
             }
           } else {
+            // moved beyond the neighbourhood: global counters (TS: the far counts, which follow
+            // the 2*ntiles counts)
+            const uint32_t fk = TS ? 2u * (uint32_t)G.ntiles : 0u;
 #pragma unroll 1
-            for (int o = 0; o < 8; ++o)  // moved beyond the neighbourhood: global counters
-              if (target_active(up, o)) atomicAdd(&counts[target_key(tn, o, G)], 1u);
+            for (int o = 0; o < 8; ++o)
+              if (target_active(up, o)) atomicAdd(&counts[target_key(tn, o, G) + fk], 1u);
           }
         }
       }
@@ -985,19 +1126,25 @@ __device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restr
   }
   if (LOCAL_HIST) {
     __syncthreads();
-    if (threadIdx.x < 128) {
-      const uint32_t cnt = hist[threadIdx.x];
-      if (cnt) {
-        const int cell = threadIdx.x >> 1, kind = threadIdx.x & 1;
-        const int rx = cell & 3, ry = (cell >> 2) & 3, rz = cell >> 4;
-        const int x = fold(tix + rx - 1, G.ntx), y = fold(tiy + ry - 1, G.nty), z = fold(tiz + rz - 1, G.ntz);
-        atomicAdd(&counts[(uint32_t)((z * G.nty + y) * G.ntx + x) + (kind ? (uint32_t)G.ntiles : 0u)], cnt);
+    // bin of this thread from the lane and the wave-uniform wave index (TS: keeps threadIdx.x
+    // from living across the kernel, which would cost a spill); the tile index likewise from
+    // values that are live anyway
+    const int bin = TS ? wv_s * 64 + (int)__lane_id() : (int)threadIdx.x;
+    if (bin < kBins) {
+      const uint32_t cnt = hist[bin];
+      uint32_t* const ctr = &counts[bin_key(tix, tiy, tiz, bin, G)];
+      if (TS) {
+        // the tile's range of the key's count space, kept for the tile scatter
+        const uint32_t t_here = (uint32_t)((tiz * G.nty + tiy) * G.ntx + tix);
+        ts_base(counts, G)[(size_t)t_here * kBins + bin] = cnt ? atomicAdd(ctr, cnt) : kNone;
+      } else if (cnt) {
+        atomicAdd(ctr, cnt);
       }
     }
   }
 }
 
-template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
+template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST, bool TS = false>
 __device__ __forceinline__ void gkd_body(const GkdState<R>& P, const R* __restrict__ phi, R kick, R drift,
                                          const Lattice& L, const TileGeom& G, const uint32_t* __restrict__ offsets,
                                          const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
@@ -1006,27 +1153,27 @@ __device__ __forceinline__ void gkd_body(const GkdState<R>& P, const R* __restri
   uint32_t beg, end, sl[kGkdE];
   tile_range(offsets, capacity, tile, beg, end);
   first_entries(entries, beg, end, L.np, sl);
-  gkd_tile<R, TX, TY, TZ, LOCAL_HIST>(P, phi, kick, drift, L, G, entries, counts, tile, beg, end, sl, box, hist,
-                                      edge);
+  gkd_tile<R, TX, TY, TZ, LOCAL_HIST, TS>(P, phi, kick, drift, L, G, entries, counts, tile, beg, end, sl, box,
+                                          hist, edge);
 }
 
 // FP32 with the LDS histogram: a register budget of 7 workgroups per CU (72 VGPRs, no spills;
 // the kernel waits on dependent loads, so the seventh workgroup pays: 482 -> 470 us at 512^3)
-template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST>
+template <typename R, int TX, int TY, int TZ, bool LOCAL_HIST, bool TS>
 __global__ __launch_bounds__(kT, sizeof(R) == 4 && LOCAL_HIST ? 7 : 1) void k_gkd_tiles(GkdState<R> P, const R* __restrict__ phi, R kick, R drift,
                                                   Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ entries, uint32_t* __restrict__ counts,
                                                   uint32_t capacity, StepTable tab,
                                                   const float4* __restrict__ edge) {
   __shared__ R box[gkd_box_cells<TX, TY, TZ>()];
-  __shared__ uint32_t hist[128];
+  __shared__ uint32_t hist[kBins];
   if (tab.table) {  // graph replay: this step's kick and drift from the device table
     const int k = *tab.step;
     kick = R(tab.table[3 * k + 1]);
     drift = R(tab.table[3 * k + 2]);
   }
-  gkd_body<R, TX, TY, TZ, LOCAL_HIST>(P, phi, kick, drift, L, G, offsets, entries, counts, capacity,
-                                      xcd_tile(blockIdx.x, gridDim.x), box, hist, edge);
+  gkd_body<R, TX, TY, TZ, LOCAL_HIST, TS>(P, phi, kick, drift, L, G, offsets, entries, counts, capacity,
+                                          xcd_tile(blockIdx.x, gridDim.x), box, hist, edge);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1038,7 +1185,7 @@ constexpr int kBT = 16;
 
 __global__ __launch_bounds__(kT) void k_batch_gkd(const BatchDesc* __restrict__ D) {
   __shared__ float box[gkd_box_cells<kBT, kBT, kBT>()];
-  __shared__ uint32_t hist[128];
+  __shared__ uint32_t hist[kBins];
   const long b = xcd_tile(blockIdx.x, gridDim.x);
   const int k = batch_member(D, BS_TILES, b);
   const BatchMember& m = D->m[k];
@@ -1159,6 +1306,17 @@ std::vector<TileGeom> tile_candidates(int n, int prec_bytes) {
 
 int scan_blocks(const TileGeom& g) { return (int)((2L * g.ntiles + kScanChunk - 1) / kScanChunk); }
 
+bool tile_scatter_ok(const TileGeom& g) {
+  return g.ntx >= 4 && g.nty >= 4 && g.ntz >= 4 && g.tx * g.ty * g.tz >= kTileScatterCells;
+}
+size_t counts_words(int n, const TileGeom& gmax) {
+  const size_t ts_tiles = (size_t)n * n * n / kTileScatterCells;  // most tiles of a tile-scatter shape
+  return std::max(4 * (size_t)gmax.ntiles, ts_old_at((long)ts_tiles) + ts_tiles + 1);
+}
+
+// the tile scatter runs when the bins hold its arrays and the shape allows it
+static bool tile_scatter_on(const TileGeom& G, const Bins& B) { return B.tile_scatter && tile_scatter_ok(G); }
+
 // tile shapes the kernels are instantiated for; make_tiles only produces these
 #define PM_TILE_DISPATCH(G, CALL)                                                                   \
   do {                                                                                              \
@@ -1179,16 +1337,31 @@ hipError_t bins_count(const PState<R>& p, const Lattice& L, const TileGeom& G, c
   return hipGetLastError();
 }
 
-hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st) {
+hipError_t bins_scan(const TileGeom& G, const Bins& B, hipStream_t st, bool after_gather) {
   const long m = 2L * G.ntiles;
   const int nb = scan_blocks(G);
-  k_scan_reduce_spine<<<nb, kT, 0, st>>>(B.counts, m, B.block_sums, nb, B.done);
-  k_scan_apply<<<nb, kT, 0, st>>>(B.counts, m, B.block_sums, B.offsets, B.cursors);
+  const bool ts = after_gather && tile_scatter_on(G, B);
+  uint32_t* far = ts ? B.counts + ts_far_at(G.ntiles) : nullptr;
+  uint32_t* old = ts ? B.counts + ts_old_at(G.ntiles) : nullptr;
+  k_scan_reduce_spine<<<nb, kT, 0, st>>>(B.counts, m, B.block_sums, nb, B.done, far);
+  k_scan_apply<<<nb, kT, 0, st>>>(B.counts, m, B.block_sums, B.offsets, B.cursors, far, old);
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t bins_scatter(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st) {
+hipError_t bins_scatter(const PState<R>& p, const Lattice& L, const TileGeom& G, const Bins& B, hipStream_t st,
+                        bool after_gather) {
+  if (after_gather && tile_scatter_on(G, B)) {
+#define PM_L(X, Y, Z)                                                                                           \
+  do {                                                                                                          \
+    if constexpr ((X) * (Y) * (Z) >= kTileScatterCells)                                                         \
+      k_tile_scatter<R, X, Y, Z><<<G.ntiles, kScatterT, 0, st>>>(p.s, p.id, L, G, B.offsets, B.cursors,         \
+                                                                 B.entries, B.capacity, B.health, B.counts);    \
+  } while (0)
+    PM_TILE_DISPATCH(G, PM_L);
+#undef PM_L
+    return hipGetLastError();
+  }
 #define PM_L(X, Y, Z)                                                                                          \
   k_bin_scatter<R, X, Y, Z><<<blocks_for(L.np), kT, 0, st>>>(p.s, p.id, L, G, B.offsets, B.cursors, B.entries, \
                                                              B.capacity, B.health)
@@ -1221,14 +1394,19 @@ hipError_t gkd_tiles(const PState<R>& in, const PState<R>& out, const R* phi, do
   if (in.s == out.s || in.v == out.v || in.id == out.id) return hipErrorInvalidValue;  // never in place
   const bool local_hist = G.ntx >= 4 && G.nty >= 4 && G.ntz >= 4;
   const GkdState<R> ps{in.s, in.v, in.id, out.s, out.v, out.id};
+  const bool ts = tile_scatter_on(G, B);
 #define PM_L(X, Y, Z)                                                                                             \
   do {                                                                                                            \
-    if (local_hist)                                                                                               \
-      k_gkd_tiles<R, X, Y, Z, true><<<G.ntiles, kT, 0, st>>>(ps, phi, R(kick), R(drift), L, G, B.offsets,         \
-                                                             B.entries, B.counts, B.capacity, tab, edge);         \
+    if (ts) {                                                                                                     \
+      if constexpr ((X) * (Y) * (Z) >= kTileScatterCells)                                                         \
+        k_gkd_tiles<R, X, Y, Z, true, true><<<G.ntiles, kT, 0, st>>>(ps, phi, R(kick), R(drift), L, G, B.offsets, \
+                                                                     B.entries, B.counts, B.capacity, tab, edge); \
+    } else if (local_hist)                                                                                        \
+      k_gkd_tiles<R, X, Y, Z, true, false><<<G.ntiles, kT, 0, st>>>(ps, phi, R(kick), R(drift), L, G, B.offsets,  \
+                                                                    B.entries, B.counts, B.capacity, tab, edge);  \
     else                                                                                                          \
-      k_gkd_tiles<R, X, Y, Z, false><<<G.ntiles, kT, 0, st>>>(ps, phi, R(kick), R(drift), L, G, B.offsets,        \
-                                                              B.entries, B.counts, B.capacity, tab, edge);        \
+      k_gkd_tiles<R, X, Y, Z, false, false><<<G.ntiles, kT, 0, st>>>(ps, phi, R(kick), R(drift), L, G, B.offsets, \
+                                                                     B.entries, B.counts, B.capacity, tab, edge); \
   } while (0)
   PM_TILE_DISPATCH(G, PM_L);
 #undef PM_L
@@ -1312,7 +1490,7 @@ hipError_t batch_scan(const BatchDesc* d, long total, int n_members, hipStream_t
   template hipError_t bins_count<R>(const PState<R>&, const Lattice&, const TileGeom&, const Bins&,         \
                                     hipStream_t);                                                            \
   template hipError_t bins_scatter<R>(const PState<R>&, const Lattice&, const TileGeom&, const Bins&,       \
-                                      hipStream_t);                                                          \
+                                      hipStream_t, bool);                                                    \
   template hipError_t deposit_tiles<R>(const PState<R>&, R*, double, const Lattice&, const TileGeom&,       \
                                        const Bins&, hipStream_t, int*, double);                              \
   template hipError_t gkd_tiles<R>(const PState<R>&, const PState<R>&, const R*, double, double,            \

@@ -42,10 +42,13 @@ def test_pure_host_entry_points():
     assert lib.pm_sim_footprint(512, 256, 32, ctypes.byref(b)) == 0
     # two particle state buffers (displacements, velocities, ids) + rho + half spectrum +
     # Nyquist plane + tile bins (8 entries per particle,
-    # counts/offsets/cursors and scan spine sized for the tile shape with the most tiles,
-    # 8^3: 2 * 64^3 counters in chunks of 1024, plus the spine's ticket counter)
+    # offsets/cursors and scan spine sized for the tile shape with the most tiles, 8^3:
+    # 2 * 64^3 counters in chunks of 1024, plus the spine's ticket counter; the counts buffer
+    # also holds the tile scatter's arrays for tiles of 2048 cells: 2 counts, 2 far counts,
+    # 128 range starts and 1 old offset per tile, + 1)
     ntiles = (512 // 8) ** 3
-    bins = 4 * (3 * 2 * ntiles + 1 + 2 * ntiles // 1024 + 1) + 4 * 8 * 256 ** 3
+    counts = 133 * 512 ** 3 // 2048 + 1
+    bins = 4 * (counts + 2 * 2 * ntiles + 1 + 2 * ntiles // 1024 + 1) + 4 * 8 * 256 ** 3
     # + a fixed 1 MiB for the step table, graphs and health counters + the gather's x-margin
     # array (a float4 per row and multiple of 32)
     assert b.value == (2 * (6 * 4 + 4) * 256 ** 3 + 2 * 512 ** 3 * 4 + 512 * 512 * 2 * 4 + bins
