@@ -533,7 +533,7 @@ size_t edge_bytes(int n) { return (size_t)n * n * (n / 32) * sizeof(float4); }
 
 size_t sim_bytes(int n, int np_side, int prec) {
   const size_t nn = (size_t)n, np = (size_t)np_side * np_side * np_side;
-  return 2 * (6 * np * prec + 4 * np) /*two state buffers*/ + nn * nn * nn * prec /*rho*/ +
+  return 2 * (6 * np * prec + 6 * np) /*two state buffers: s, v, ids + bin codes*/ + nn * nn * nn * prec /*rho*/ +
          nn * nn * nn * prec /*H*/ + nn * nn * 2 * prec /*Q*/ + bins_bytes(n, np_side, prec) + kSimOverhead +
          (has_edge(n, prec) ? edge_bytes(n) : 0);
 }
@@ -1049,7 +1049,7 @@ PM_API int pm_sim_create(int n_cells, int n_parts_side, int precision, unsigned 
   for (int k = 0; k < 2; ++k)
     ok = ok && hipMalloc(&s->sb[k], 3 * (size_t)s->L.np * pr) == hipSuccess &&
          hipMalloc(&s->vb[k], 3 * (size_t)s->L.np * pr) == hipSuccess &&
-         hipMalloc(&s->idb[k], (size_t)s->L.np * 4) == hipSuccess;
+         hipMalloc(&s->idb[k], (size_t)s->L.np * 6) == hipSuccess;  // ids, then the tile scatter's bin codes
   if (!ok || hipMalloc(&s->grid[0], nn * nn * nn * pr) != hipSuccess ||
       hipMalloc(&s->grid[1], nn * nn * nn * pr) != hipSuccess || hipMalloc(&s->Q, nn * nn * 2 * pr) != hipSuccess ||
       (has_edge(n_cells, s->prec) && hipMalloc(&s->edge, edge_bytes(n_cells)) != hipSuccess))

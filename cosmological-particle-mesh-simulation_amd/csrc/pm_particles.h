@@ -98,7 +98,7 @@ struct Bins {
   // Tile scatter after a gather (false: the particle-parallel scatter with device cursors).
   // Its arrays follow the 2*ntiles counts of the current shape in the counts buffer (sized by
   // counts_words): 2*ntiles far counts (zero between binnings, like the counts), 128 reserved
-  // range starts per tile, the previous binning's ntiles + 1 own offsets.
+  // range starts per tile, the previous binning's ntiles + 1 own offsets, a bin code per particle.
   bool tile_scatter = false;
 };
 
@@ -114,12 +114,13 @@ int scan_blocks(const TileGeom& g);
 constexpr int kTileScatterCells = 2048;
 bool tile_scatter_ok(const TileGeom& g);
 // words of the counts buffer on an n^3 grid whose tile shape with most tiles is gmax: the counts
-// and far counts (4 per tile of any shape) and the tile scatter's 133 words per tile (+ 1) for
+// and far counts (4 per tile of any shape), and the tile scatter's 133 words per tile (+ 1) for
 // every shape it runs on
 size_t counts_words(int n, const TileGeom& gmax);
 
 // Resident particle state: displacements s and velocities v (3 x N_p each, component-major)
-// and the id (lattice slot) of the particle at each storage index.  A sim holds two of them;
+// and the id (lattice slot) of the particle at each storage index (the id buffer holds N_p
+// 16-bit bin codes after the ids, written by the gather for the tile scatter).  A sim holds two of them;
 // the gather reads one and writes the other in tile-bin order.
 template <typename R> struct PState {
   R* s;

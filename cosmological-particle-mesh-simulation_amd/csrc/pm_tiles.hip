@@ -37,6 +37,10 @@ inline unsigned blocks_for(long n, int b = kT) {
 }
 
 
+template <typename R> __device__ __forceinline__ bool is_finite3(R a, R b, R c) {
+  return __builtin_isfinite(a) && __builtin_isfinite(b) && __builtin_isfinite(c);
+}
+
 // index in [-n, 2n) folded into [0, n)
 __device__ __forceinline__ int fold(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -128,15 +132,20 @@ __device__ __forceinline__ uint32_t bin_key(int tix, int tiy, int tiz, int bin, 
 constexpr int kBins = 128;  // neighbourhood bins per source tile
 // Tile scatter arrays, laid out in the counts buffer after the 2*ntiles counts of the current
 // tile shape: 2*ntiles far counts (particles that left their source tile's neighbourhood), 128
-// reserved range starts per tile (kNone: empty bin), and the own offsets of the binning the
-// gather read (ntiles + 1; the scan keeps them before it overwrites them): the source range of
-// tile t, where the gather wrote its particles, is [old[t], old[t+1]).
+// reserved range starts per tile (kNone: empty bin), the own offsets of the binning the gather
+// read (ntiles + 1; the scan keeps them before it overwrites them: the source range of tile t,
+// where the gather wrote its particles, is [old[t], old[t+1])).  The 16-bit bin code of each
+// stored particle follows the N_p ids of the state buffer.
 __host__ __device__ __forceinline__ size_t ts_far_at(long ntiles) { return 2 * (size_t)ntiles; }
 __host__ __device__ __forceinline__ size_t ts_base_at(long ntiles) { return 4 * (size_t)ntiles; }
 __host__ __device__ __forceinline__ size_t ts_old_at(long ntiles) { return (4 + (size_t)kBins) * ntiles; }
 __device__ __forceinline__ uint32_t* ts_base(uint32_t* counts, const TileGeom& G) {
   return counts + ts_base_at(G.ntiles);
 }
+// Bin code of a particle the gather moved: the neighbourhood cell of its new home tile (h0 / 2,
+// bits 0-5) and its upper-face flags (bits 6-8), or kCodeFar when it moved beyond the
+// neighbourhood.
+constexpr uint32_t kCodeFar = 0x8000u;
 constexpr int kScatterT = 64;  // threads per block of the tile scatter (one wavefront per tile)
 constexpr int kScatterE = 4;   // particles per thread and batch in the tile scatter
 
@@ -255,17 +264,11 @@ __device__ __forceinline__ void bin_scatter_body(const R* __restrict__ s, const 
   const bool valid = slot < L.np;
   int c[3] = {0, 0, 0}, t[3] = {0, 0, 0}, up[3] = {0, 0, 0};
   R d[3];
-  bool finite = true;
   if (valid) {
-    const R x0 = s[slot], x1 = s[L.np + slot], x2 = s[2 * L.np + slot];
-    finite = __builtin_isfinite(x0) && __builtin_isfinite(x1) && __builtin_isfinite(x2);
-    R q[3];
-    cell_and_frac<R>(x0, x1, x2, id[slot], L, c, d, q);
+    // a non-finite displacement is binned at the clamped cell (the deposit counts it)
+    cell_and_frac<R>(s, slot, id[slot], L, c, d);
     tile_of<TX, TY, TZ>(c, G, t, up);
   }
-  // a non-finite displacement (from a non-finite velocity in the drift, too) is binned at the
-  // clamped cell and counted
-  if (!finite && health) atomicAdd(&health[PM_HEALTH_NONFINITE], 1u);
   // halo keys only in wavefronts that have any (wave-uniform)
   const bool any_halo = __ballot(valid && (up[0] | up[1] | up[2])) != 0ull;
   uint32_t key[8], beg[8], lim[8];
@@ -314,15 +317,15 @@ __global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, con
   bin_scatter_body<R, TX, TY, TZ>(s, id, L, G, offsets, cursors, entries, capacity, blockIdx.x, health);
 }
 
-// Scatter after a gather, one workgroup per source tile: the particles the gather of tile t
-// wrote (storage [srange[2t], srange[2t+1]) of its output) are placed through LDS cursors.  The
-// gather reserved, with the one counter atomic per (tile, neighbourhood bin) it issues anyway, a
-// range of every key's count space for the tile's particles (wbase; kNone for empty bins); the
-// cursor of a bin starts at offsets[key] + wbase, so the places cost LDS atomics, not device
-// atomics.  Particles that moved beyond the neighbourhood were counted in `far`; the scan put
-// them after the reserved ranges of each key, and they take a device cursor atomic.  The bins and
-// the neighbourhood test are recomputed from the stored positions, bit for bit those of the
-// gather.
+// Scatter after a gather, one wavefront per source tile: the particles the gather of tile t
+// wrote (storage [old[t], old[t+1]) of its output) are placed through LDS cursors.  The gather
+// reserved, with the one counter atomic per (tile, neighbourhood bin) it issues anyway, a range
+// of every key's count space for the tile's particles (wbase; kNone for empty bins); the cursor
+// of a bin starts at offsets[key] + wbase, so the places cost LDS atomics, not device atomics.
+// The gather also left each particle's bin code (2 bytes instead of the 16 of its displacement
+// and id).  Particles that moved beyond the neighbourhood were counted in the far counts; the
+// scan put them after the reserved ranges of each key, and they take a device cursor atomic on
+// keys recomputed from their stored state.
 template <typename R, int TX, int TY, int TZ>
 __global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict__ s, const uint32_t* __restrict__ id,
                                                             Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
@@ -333,7 +336,9 @@ __global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict_
   constexpr int NT = kScatterT, E = kScatterE, BPT = kBins / NT;
   static_assert(kBins % NT == 0, "whole bins per thread");
   __shared__ uint32_t cur[kBins], lim[kBins];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  // each XCD walks its run of tiles backwards, so it starts on what the gather (walking the same
+  // runs forward) wrote last and the last-level cache still holds
+  const int tile = xcd_block(blockIdx.x, gridDim.x, 1);
   const long np = L.np;
   const uint32_t* wbase = ts_base(counts, G);
   const uint32_t* old = counts + ts_old_at(G.ntiles);
@@ -344,16 +349,13 @@ __global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict_
   uint32_t w[BPT];
 #pragma unroll
   for (int k = 0; k < BPT; ++k) w[k] = wbase[(size_t)tile * kBins + k * NT + threadIdx.x];
-  R sv[E][3];
-  uint32_t pid[E];
+  const uint16_t* codes = reinterpret_cast<const uint16_t*>(id) + 2 * np;
+  uint32_t code[E];
   auto load = [&](uint32_t base) {
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const uint32_t e = base + u * NT + threadIdx.x;
-      const bool ok = e < end;
-      pid[u] = ok ? id[e] : 0u;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) sv[u][a] = ok ? s[a * np + e] : R(0);
+      code[u] = e < end ? (uint32_t)codes[e] : 0u;
     }
   };
   load(beg);
@@ -370,20 +372,16 @@ __global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict_
     lim[b] = l;
   }
   __syncthreads();
-  uint32_t lost = 0, nonfinite = 0;
+  uint32_t lost = 0;
   for (uint32_t base = beg; base < end; base += NT * E) {
     if (base != beg) load(base);
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const uint32_t e = base + u * NT + threadIdx.x;
       if (e >= end) continue;
-      nonfinite += (__builtin_isfinite(sv[u][0]) && __builtin_isfinite(sv[u][1]) && __builtin_isfinite(sv[u][2])) ? 0u : 1u;
-      int c[3], tn[3], up[3];
-      R d[3], q[3];
-      cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], pid[u], L, c, d, q);
-      tile_of<TX, TY, TZ>(c, G, tn, up);
-      int h0;
-      if (near_home_bin(tn, tc, G, h0)) {
+      if (!(code[u] & kCodeFar)) {
+        const int h0 = 2 * (int)(code[u] & 63u);
+        const int up[3] = {(int)(code[u] >> 6) & 1, (int)(code[u] >> 7) & 1, (int)(code[u] >> 8) & 1};
 #pragma unroll
         for (int o = 0; o < 8; ++o) {
           if (!target_active(up, o)) continue;
@@ -393,6 +391,11 @@ __global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict_
           else ++lost;
         }
       } else {
+        // moved beyond the neighbourhood (rare): its keys from the stored state
+        int c[3], tn[3], up[3];
+        R d[3];
+        cell_and_frac<R>(s, e, id[e], L, c, d);
+        tile_of<TX, TY, TZ>(c, G, tn, up);
 #pragma unroll 1
         for (int o = 0; o < 8; ++o) {
           if (!target_active(up, o)) continue;
@@ -404,7 +407,6 @@ __global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict_
       }
     }
   }
-  if (nonfinite && health) atomicAdd(&health[PM_HEALTH_NONFINITE], nonfinite);
   if (lost && health) atomicAdd(&health[PM_HEALTH_OVERFLOW], lost);
 }
 
@@ -595,7 +597,8 @@ __device__ __forceinline__ void deposit_body(const R* __restrict__ s, const uint
     else acc[i] = 0.0;
   }
   __syncthreads();
-  uint32_t bad = 0;  // own entries whose cell lies outside the tile (stale bins)
+  uint32_t bad = 0;        // own entries whose cell lies outside the tile (stale bins)
+  uint32_t nonfinite = 0;  // own entries with a non-finite displacement (binned at a clamped cell)
   for (uint32_t base = 0; base < total; base += kT * E) {
     if (base != 0) load(base);
 #pragma unroll
@@ -604,8 +607,10 @@ __device__ __forceinline__ void deposit_body(const R* __restrict__ s, const uint
       int c[3];
       R d[3], q[3];
       cell_and_frac<R>(sv[u][0], sv[u][1], sv[u][2], pid[u], L, c, d, q);
-      if (base + u * kT + threadIdx.x < n_own)
+      if (base + u * kT + threadIdx.x < n_own) {  // each particle is an own entry once
         bad += (c[0] / TX != tix || c[1] / TY != tiy || c[2] / TZ != tiz) ? 1u : 0u;
+        nonfinite += is_finite3(sv[u][0], sv[u][1], sv[u][2]) ? 0u : 1u;
+      }
       int r0[3], r1[3];  // tile-relative coordinates of the two corners per axis (-1: outside)
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -646,6 +651,7 @@ __device__ __forceinline__ void deposit_body(const R* __restrict__ s, const uint
     }
   }
   if (bad && health) atomicAdd(&health[PM_HEALTH_BINS], bad);
+  if (nonfinite && health) atomicAdd(&health[PM_HEALTH_NONFINITE], nonfinite);
   __syncthreads();
   const size_t nn = (size_t)n;
 #pragma unroll
@@ -717,7 +723,8 @@ __global__ __launch_bounds__(kDrT) void k_deposit_r2c(const float* __restrict__ 
   for (int i = tid; i < M * SN; i += NT) lds[i] = make_float2(0.f, 0.f);
   const long np = L.np;
   const int n = L.n;
-  uint32_t bad = 0;  // own entries whose cell lies outside the row block (stale bins)
+  uint32_t bad = 0;        // own entries whose cell lies outside the row block (stale bins)
+  uint32_t nonfinite = 0;  // own entries with a non-finite displacement
   __syncthreads();
   for (uint32_t base = 0; base < total; base += NT * E) {
     uint32_t sl[E], pid[E];
@@ -747,6 +754,7 @@ __global__ __launch_bounds__(kDrT) void k_deposit_r2c(const float* __restrict__ 
       if (i < n_own) {
         tx = c[0] / TX;
         bad += (c[1] / TY != ty || c[2] / TZ != tz) ? 1u : 0u;
+        nonfinite += is_finite3(sv[u][0], sv[u][1], sv[u][2]) ? 0u : 1u;
       } else {
         const uint32_t e = hb + (i - n_own);
         int lo = 0, hi = NTX - 1;
@@ -795,6 +803,7 @@ __global__ __launch_bounds__(kDrT) void k_deposit_r2c(const float* __restrict__ 
     }
   }
   if (bad && health) atomicAdd(&health[PM_HEALTH_BINS], bad);
+  if (nonfinite && health) atomicAdd(&health[PM_HEALTH_NONFINITE], nonfinite);
   __syncthreads();
   // rows along x: N/2-point complex FFT, then the even/odd untangling (as r2c_body)
   fft_passes<float2, M, 1, -1, RB, SN, NT, 2>(lds, tw, tid);
@@ -1098,7 +1107,13 @@ __device__ __forceinline__ void gkd_tile(const GkdState<R>& ps, const R* __restr
         if (LOCAL_HIST) {
           const int tc[3] = {tix, tiy, tiz};
           int h0;
-          if (near_home_bin(tn, tc, G, h0)) {
+          const bool near = near_home_bin(tn, tc, G, h0);
+          if (TS) {  // the particle's bins for the tile scatter
+            uint32_t code = near ? (uint32_t)(h0 >> 1) | (uint32_t)(up[0] | (up[1] << 1) | (up[2] << 2)) << 6 : kCodeFar;
+            // after the ids of the output buffer (addressed from the id pointer: no extra base)
+            reinterpret_cast<uint16_t*>(ps.id_out)[2 * np + eo] = (uint16_t)code;
+          }
+          if (near) {
             atomicAdd(&hist[h0], 1u);
             if (up[0] | up[1] | up[2]) {
 #pragma unroll

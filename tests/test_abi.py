@@ -40,8 +40,8 @@ def test_pure_host_entry_points():
     assert abs(lib.pm_f(0.5, 0.3, 0.7, 0.0) - 1 / (0.5 * (0.3 / 0.125 + 0.7) ** 0.5)) < 1e-14
     b = ctypes.c_size_t()
     assert lib.pm_sim_footprint(512, 256, 32, ctypes.byref(b)) == 0
-    # two particle state buffers (displacements, velocities, ids) + rho + half spectrum +
-    # Nyquist plane + tile bins (8 entries per particle,
+    # two particle state buffers (displacements, velocities, ids + 16-bit bin codes) + rho +
+    # half spectrum + Nyquist plane + tile bins (8 entries per particle,
     # offsets/cursors and scan spine sized for the tile shape with the most tiles, 8^3:
     # 2 * 64^3 counters in chunks of 1024, plus the spine's ticket counter; the counts buffer
     # also holds the tile scatter's arrays for tiles of 2048 cells: 2 counts, 2 far counts,
@@ -51,7 +51,7 @@ def test_pure_host_entry_points():
     bins = 4 * (counts + 2 * 2 * ntiles + 1 + 2 * ntiles // 1024 + 1) + 4 * 8 * 256 ** 3
     # + a fixed 1 MiB for the step table, graphs and health counters + the gather's x-margin
     # array (a float4 per row and multiple of 32)
-    assert b.value == (2 * (6 * 4 + 4) * 256 ** 3 + 2 * 512 ** 3 * 4 + 512 * 512 * 2 * 4 + bins
+    assert b.value == (2 * (6 * 4 + 4 + 2) * 256 ** 3 + 2 * 512 ** 3 * 4 + 512 * 512 * 2 * 4 + bins
                        + (1 << 20) + 512 * 512 * 16 * 16)
     assert lib.pm_sim_footprint(512, 256, 16, ctypes.byref(b)) != 0
     assert lib.pm_supported_grid(512) == 1 and lib.pm_supported_grid(100) == 0

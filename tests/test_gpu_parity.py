@@ -543,6 +543,38 @@ def test_backward_x_pass_traversal_gives_the_bits_of_forward(n, npart, prec, til
         assert np.array_equal(a_, b_), name
 
 
+# ------------------------------------------------------------------ particles that move far
+def test_particles_beyond_the_tile_neighbourhood_are_placed():
+    # The tile scatter places a source tile's particles through ranges the gather reserved in
+    # the keys of its 4^3 tile neighbourhood; a particle whose new tile lies two or more tiles
+    # away is counted apart and placed after those ranges.  Two percent of the particles get
+    # +-40 per axis of extra velocity (about 25 cells per step, three 8-cell tiles in y and z):
+    # with the order-independent deposit, three steps in 32 x 8 x 8 tiles (tile scatter) give
+    # the bits of three steps in 8^3 tiles (particle-parallel scatter).
+    from pmgpu.sim import PMSim
+    n, npart = 256, 64
+    pos, vel = make_ics(n, npart, seed=83, box=float(2 * n))
+    rng = np.random.default_rng(83)
+    sel = rng.choice(npart ** 3, npart ** 3 // 50, replace=False)
+    vel[:, sel] += rng.choice(np.float32([-40.0, 40.0]), size=(3, sel.size))
+    kw = dict(omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+    out = []
+    for tiles in ((32, 8, 8), (8, 8, 8)):
+        s = PMSim(n, npart, 32, deterministic=True, **kw)
+        s.set_tiles(*tiles)
+        s.set_state(pos, vel)
+        x0, _ = s.get_state(np.float64)
+        s.step(0.05, 0.01)
+        x1, _ = s.get_state(np.float64)
+        moved = np.abs(periodic_diff(x1[:, sel], x0[:, sel], n))
+        assert (moved[1:] > 17).mean() > 0.9  # beyond the neighbourhood along y and z
+        s.run(0.06, 0.01, 2)
+        out.append((*s.get_state(np.float64), s.density(np.float64)))
+        s.close()
+    for name, a_, b_ in zip(("x", "v", "rho"), out[0], out[1]):
+        assert np.array_equal(a_, b_), name
+
+
 # ------------------------------------------------------------------ corrupt state detection
 def test_nonfinite_state_is_reported():
     from pmgpu import _lib
