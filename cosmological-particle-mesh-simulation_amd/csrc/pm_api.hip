@@ -1539,6 +1539,7 @@ void ens_describe(pm_ensemble* e) {
     m.from_grid = s->fgrid ? 1 : 0;
     m.fgrid = s->fgrid;
     m.local_hist = (s->G.ntx >= 4 && s->G.nty >= 4 && s->G.ntz >= 4) ? 1 : 0;
+    m.tile_scatter = (s->B.tile_scatter && tile_scatter_ok(s->G)) ? 1 : 0;
     for (int b = 0; b < 2; ++b) {
       m.s[b] = (float*)s->sb[b];
       m.v[b] = (float*)s->vb[b];
@@ -1565,8 +1566,11 @@ void ens_describe(pm_ensemble* e) {
     blocks[BS_Z] = batch_fft_blocks(n, BS_Z);
     blocks[BS_C2R] = batch_fft_blocks(n, BS_C2R);
     blocks[BS_TILES] = s->G.ntiles;
+    blocks[BS_GKD] = m.tile_scatter ? 0 : s->G.ntiles;
+    blocks[BS_GKD_TS] = m.tile_scatter ? s->G.ntiles : 0;
     blocks[BS_SCAN] = m.scan_nb;
-    blocks[BS_SCATTER] = batch_scatter_blocks(s->L.np);
+    blocks[BS_SCATTER] = m.tile_scatter ? 0 : batch_scatter_blocks(s->L.np);
+    blocks[BS_TILE_SCATTER] = m.tile_scatter ? s->G.ntiles : 0;
     const int c = batch_class(n);
     for (int st = 0; st < BS_COUNT; ++st) {
       D.off[st][i + 1] = D.off[st][i] + blocks[st];
@@ -1620,12 +1624,12 @@ int ens_step(pm_ensemble* e) {
   }
   {
     EnsTimer t(e, ST_GKD);
-    PM_HIP(batch_gkd(d, e->total[BS_TILES], st));
+    PM_HIP(batch_gkd(d, e->total[BS_GKD], e->total[BS_GKD_TS], st));
   }
   {
     EnsTimer t(e, ST_BIN);
     PM_HIP(batch_scan(d, e->total[BS_SCAN], (int)e->members.size(), st));
-    PM_HIP(batch_scatter(d, e->total[BS_SCATTER], st));
+    PM_HIP(batch_scatter(d, e->total[BS_SCATTER], e->total[BS_TILE_SCATTER], st));
   }
   {
     EnsTimer t(e, ST_DEPOSIT);

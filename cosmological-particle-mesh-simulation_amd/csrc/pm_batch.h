@@ -15,7 +15,11 @@ namespace pm {
 constexpr int kBatchMax = 64;  // members per ensemble
 
 // stages whose block count differs per member; the forward and inverse y passes share BS_Y
-enum BatchStage { BS_R2C = 0, BS_Y, BS_Z, BS_C2R, BS_TILES, BS_SCAN, BS_SCATTER, BS_COUNT };
+// (BS_TILES: all tiles, for the deposit; the gather's tiles split by whether the member takes the
+// tile scatter, BS_GKD_TS, or not, BS_GKD, and so do the scatters)
+enum BatchStage {
+  BS_R2C = 0, BS_Y, BS_Z, BS_C2R, BS_TILES, BS_GKD, BS_GKD_TS, BS_SCAN, BS_SCATTER, BS_TILE_SCATTER, BS_COUNT
+};
 
 struct BatchMember {
   // grids (FP32): density, half spectrum (overwritten by the potential) and Nyquist plane
@@ -29,7 +33,8 @@ struct BatchMember {
   int n;
   int deconvolve;
   int from_grid;
-  int local_hist;  // the gather's LDS neighbourhood histogram applies (>= 4 tiles per axis)
+  int local_hist;    // the gather's LDS neighbourhood histogram applies (>= 4 tiles per axis)
+  int tile_scatter;  // the gather reserves the tile scatter's ranges (tile_scatter_ok)
   const float* fgrid;
   // particles: the two state buffers (displacements, velocities: 3 x N_p each; ids), the one
   // holding the current state at step 0 of the call (par), tile bins
@@ -93,9 +98,12 @@ hipError_t batch_r2c(const BatchDesc* d, int cls, long base, long count, int lds
 // mode -1: forward y, 0: forward z + k-space operator + inverse z, +1: inverse y
 hipError_t batch_pencil(const BatchDesc* d, int cls, int mode, long base, long count, int lds, hipStream_t st);
 hipError_t batch_c2r(const BatchDesc* d, int cls, long base, long count, int lds, hipStream_t st);
-hipError_t batch_gkd(const BatchDesc* d, long total, hipStream_t st);
+// total: tiles of the members without the tile scatter (BS_GKD), total_ts: with it (BS_GKD_TS)
+hipError_t batch_gkd(const BatchDesc* d, long total, long total_ts, hipStream_t st);
 hipError_t batch_scan(const BatchDesc* d, long total, int n_members, hipStream_t st);
-hipError_t batch_scatter(const BatchDesc* d, long total, hipStream_t st);
+// total: blocks of the particle-parallel scatter (BS_SCATTER), total_tiles: wavefronts of the
+// tile scatter (BS_TILE_SCATTER)
+hipError_t batch_scatter(const BatchDesc* d, long total, long total_tiles, hipStream_t st);
 hipError_t batch_deposit(const BatchDesc* d, long total, hipStream_t st);
 long batch_scatter_blocks(long np);
 

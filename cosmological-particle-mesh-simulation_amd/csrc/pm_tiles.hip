@@ -327,18 +327,14 @@ __global__ __launch_bounds__(kT) void k_bin_scatter(const R* __restrict__ s, con
 // scan put them after the reserved ranges of each key, and they take a device cursor atomic on
 // keys recomputed from their stored state.
 template <typename R, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict__ s, const uint32_t* __restrict__ id,
-                                                            Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
-                                                            uint32_t* __restrict__ cursors,
-                                                            uint32_t* __restrict__ entries, uint32_t capacity,
-                                                            uint32_t* __restrict__ health,
-                                                            uint32_t* __restrict__ counts) {
+__device__ __forceinline__ void tile_scatter_body(const R* __restrict__ s, const uint32_t* __restrict__ id,
+                                                  const Lattice& L, const TileGeom& G,
+                                                  const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursors,
+                                                  uint32_t* __restrict__ entries, uint32_t capacity,
+                                                  uint32_t* __restrict__ health, uint32_t* __restrict__ counts,
+                                                  int tile, uint32_t* cur, uint32_t* lim) {
   constexpr int NT = kScatterT, E = kScatterE, BPT = kBins / NT;
   static_assert(kBins % NT == 0, "whole bins per thread");
-  __shared__ uint32_t cur[kBins], lim[kBins];
-  // each XCD walks its run of tiles backwards, so it starts on what the gather (walking the same
-  // runs forward) wrote last and the last-level cache still holds
-  const int tile = xcd_block(blockIdx.x, gridDim.x, 1);
   const long np = L.np;
   const uint32_t* wbase = ts_base(counts, G);
   const uint32_t* old = counts + ts_old_at(G.ntiles);
@@ -408,6 +404,20 @@ __global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict_
     }
   }
   if (lost && health) atomicAdd(&health[PM_HEALTH_OVERFLOW], lost);
+}
+
+template <typename R, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kScatterT) void k_tile_scatter(const R* __restrict__ s, const uint32_t* __restrict__ id,
+                                                            Lattice L, TileGeom G, const uint32_t* __restrict__ offsets,
+                                                            uint32_t* __restrict__ cursors,
+                                                            uint32_t* __restrict__ entries, uint32_t capacity,
+                                                            uint32_t* __restrict__ health,
+                                                            uint32_t* __restrict__ counts) {
+  __shared__ uint32_t cur[kBins], lim[kBins];
+  // each XCD walks its run of tiles backwards, so it starts on what the gather (walking the same
+  // runs forward) wrote last and the last-level cache still holds
+  tile_scatter_body<R, TX, TY, TZ>(s, id, L, G, offsets, cursors, entries, capacity, health, counts,
+                                   xcd_block(blockIdx.x, gridDim.x, 1), cur, lim);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1198,19 +1208,26 @@ __global__ __launch_bounds__(kT, sizeof(R) == 4 && LOCAL_HIST ? 7 : 1) void k_gk
 // ------------------------------------------------------------------------------------------
 constexpr int kBT = 16;
 
-__global__ __launch_bounds__(kT) void k_batch_gkd(const BatchDesc* __restrict__ D) {
+// TS: the members with the tile scatter (stage BS_GKD_TS), otherwise the others (BS_GKD); each
+// launch holds one code path, so the tile scatter's reservations cost the others no registers
+template <bool TS>
+__global__ __launch_bounds__(kT, TS ? 5 : 1) void k_batch_gkd(const BatchDesc* __restrict__ D) {
   __shared__ float box[gkd_box_cells<kBT, kBT, kBT>()];
   __shared__ uint32_t hist[kBins];
+  constexpr int stage = TS ? BS_GKD_TS : BS_GKD;
   const long b = xcd_tile(blockIdx.x, gridDim.x);
-  const int k = batch_member(D, BS_TILES, b);
+  const int k = batch_member(D, stage, b);
   const BatchMember& m = D->m[k];
-  const int tile = (int)(b - D->off[BS_TILES][k]);
+  const int tile = (int)(b - D->off[stage][k]);
   const int step = *D->step;
   const double* sc = D->table + 3 * ((long)step * D->n_members + k);
   const float kick = float(sc[1]), drift = float(sc[2]);
   const int in = (m.par + step) & 1;
   const GkdState<float> ps{m.s[in], m.v[in], m.id[in], m.s[in ^ 1], m.v[in ^ 1], m.id[in ^ 1]};
-  if (m.local_hist)
+  if (TS)
+    gkd_body<float, kBT, kBT, kBT, true, true>(ps, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
+                                               m.offsets, m.entries, m.counts, m.capacity, tile, box, hist);
+  else if (m.local_hist)
     gkd_body<float, kBT, kBT, kBT, true>(ps, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
                                          m.offsets, m.entries, m.counts, m.capacity, tile, box, hist);
   else
@@ -1239,6 +1256,9 @@ __global__ __launch_bounds__(kT) void k_batch_deposit(const BatchDesc* __restric
                                               m.capacity, tile, acc, m.health);
 }
 
+// particle-parallel scatter of the members without the tile scatter (BS_SCATTER has no blocks
+// for the others), and the tile scatter of those with it (BS_TILE_SCATTER: one wavefront per
+// tile, each XCD walking its run backwards as in the single-simulation launch)
 __global__ __launch_bounds__(kT) void k_batch_scatter(const BatchDesc* __restrict__ D) {
   const long b = blockIdx.x;
   const int k = batch_member(D, BS_SCATTER, b);
@@ -1248,6 +1268,17 @@ __global__ __launch_bounds__(kT) void k_batch_scatter(const BatchDesc* __restric
                                          b - D->off[BS_SCATTER][k], m.health);
 }
 
+__global__ __launch_bounds__(kScatterT) void k_batch_tile_scatter(const BatchDesc* __restrict__ D) {
+  __shared__ uint32_t cur_bins[kBins], lim_bins[kBins];
+  const long b = xcd_block(blockIdx.x, gridDim.x, 1);
+  const int k = batch_member(D, BS_TILE_SCATTER, b);
+  const BatchMember& m = D->m[k];
+  const int cur = batch_state(D, m);
+  tile_scatter_body<float, kBT, kBT, kBT>(m.s[cur], m.id[cur], m.L, m.G, m.offsets, m.cursors, m.entries,
+                                          m.capacity, m.health, m.counts, (int)(b - D->off[BS_TILE_SCATTER][k]),
+                                          cur_bins, lim_bins);
+}
+
 __global__ __launch_bounds__(kT) void k_batch_scan_reduce(const BatchDesc* __restrict__ D) {
   __shared__ uint32_t sh[kT];
   __shared__ uint32_t last;
@@ -1255,7 +1286,7 @@ __global__ __launch_bounds__(kT) void k_batch_scan_reduce(const BatchDesc* __res
   const int k = batch_member(D, BS_SCAN, b);
   const BatchMember& m = D->m[k];
   scan_reduce_spine_body(m.counts, 2L * m.G.ntiles, m.block_sums, m.scan_nb, m.scan_done, b - D->off[BS_SCAN][k],
-                         sh, &last);
+                         sh, &last, m.tile_scatter ? m.counts + ts_far_at(m.G.ntiles) : nullptr);
 }
 
 __global__ __launch_bounds__(kT) void k_batch_scan_apply(const BatchDesc* __restrict__ D) {
@@ -1263,7 +1294,9 @@ __global__ __launch_bounds__(kT) void k_batch_scan_apply(const BatchDesc* __rest
   const long b = blockIdx.x;
   const int k = batch_member(D, BS_SCAN, b);
   const BatchMember& m = D->m[k];
-  scan_apply_body(m.counts, 2L * m.G.ntiles, m.block_sums, m.offsets, m.cursors, b - D->off[BS_SCAN][k], sh);
+  scan_apply_body(m.counts, 2L * m.G.ntiles, m.block_sums, m.offsets, m.cursors, b - D->off[BS_SCAN][k], sh,
+                  m.tile_scatter ? m.counts + ts_far_at(m.G.ntiles) : nullptr,
+                  m.tile_scatter ? m.counts + ts_old_at(m.G.ntiles) : nullptr);
   // the last kernel that reads the step's scalars was the gather: advance the counter here, so
   // the binning and the deposit find the gather's output buffer from it
   if (b == 0 && threadIdx.x == 0) *D->step += 1;
@@ -1471,11 +1504,16 @@ static hipError_t batch_grid(long total, unsigned& g) {
   return hipSuccess;
 }
 
-hipError_t batch_gkd(const BatchDesc* d, long total, hipStream_t st) {
+hipError_t batch_gkd(const BatchDesc* d, long total, long total_ts, hipStream_t st) {
   unsigned g;
-  if (total <= 0) return hipSuccess;
-  if (hipError_t e = batch_grid(total, g)) return e;
-  k_batch_gkd<<<g, kT, 0, st>>>(d);
+  if (total > 0) {
+    if (hipError_t e = batch_grid(total, g)) return e;
+    k_batch_gkd<false><<<g, kT, 0, st>>>(d);
+  }
+  if (total_ts > 0) {
+    if (hipError_t e = batch_grid(total_ts, g)) return e;
+    k_batch_gkd<true><<<g, kT, 0, st>>>(d);
+  }
   return hipGetLastError();
 }
 hipError_t batch_deposit(const BatchDesc* d, long total, hipStream_t st) {
@@ -1485,11 +1523,16 @@ hipError_t batch_deposit(const BatchDesc* d, long total, hipStream_t st) {
   k_batch_deposit<<<g, kT, 0, st>>>(d);
   return hipGetLastError();
 }
-hipError_t batch_scatter(const BatchDesc* d, long total, hipStream_t st) {
+hipError_t batch_scatter(const BatchDesc* d, long total, long total_tiles, hipStream_t st) {
   unsigned g;
-  if (total <= 0) return hipSuccess;
-  if (hipError_t e = batch_grid(total, g)) return e;
-  k_batch_scatter<<<g, kT, 0, st>>>(d);
+  if (total > 0) {
+    if (hipError_t e = batch_grid(total, g)) return e;
+    k_batch_scatter<<<g, kT, 0, st>>>(d);
+  }
+  if (total_tiles > 0) {
+    if (hipError_t e = batch_grid(total_tiles, g)) return e;
+    k_batch_tile_scatter<<<g, kScatterT, 0, st>>>(d);
+  }
   return hipGetLastError();
 }
 hipError_t batch_scan(const BatchDesc* d, long total, int n_members, hipStream_t st) {
