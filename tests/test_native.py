@@ -1,8 +1,9 @@
 """Sanitizer runs of the host restatement of the tile binning (tests/native/bin_restatement.cpp).
 
 The restatement follows the binning kernels' index arithmetic (keys, segments, cursor places,
-the owner-computes deposit and the fused kernel's row-block halo search) with std::thread
-workers, and checks its invariants against a direct CIC deposit.  It is built twice with g++:
+the owner-computes deposit, the fused kernel's row-block halo search, and the tile scatter's
+neighbourhood bins, reserved ranges, far counts and bin codes after a move of every particle)
+with std::thread workers, and checks its invariants against a direct CIC deposit.  It is built twice with g++:
 with AddressSanitizer + UndefinedBehaviorSanitizer (out-of-bounds indices, overflow) and with
 ThreadSanitizer (races between workers: two tiles writing one cell would be one).
 """
@@ -29,4 +30,4 @@ def test_binning_restatement_under_sanitizer(tmp_path, san):
     p = subprocess.run([str(exe)], capture_output=True, text=True, env=env, timeout=300)
     print(p.stdout)
     assert p.returncode == 0, p.stdout + p.stderr
-    assert "FAILED" not in p.stdout and p.stdout.count(": ok") == 7
+    assert "FAILED" not in p.stdout and p.stdout.count(": ok") == 12
