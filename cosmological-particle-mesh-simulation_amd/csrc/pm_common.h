@@ -88,6 +88,14 @@ template <typename R> __host__ __device__ __forceinline__ R wrap_disp(R s, R n) 
 // its own L2; this maps dispatch index b of an nblocks-block launch to a logical block so that
 // every XCD works through one contiguous run of logical blocks, and neighbours that share cache
 // lines (adjacent tiles, adjacent column chunks of an FFT pass) use one L2.
+// A pointer read from device memory (a descriptor) recast as a global-memory pointer: the
+// compiler cannot tell such a pointer from one into LDS or scratch and emits FLAT accesses,
+// which also count against the LDS wait counter (so waiting on LDS waits on memory too); through
+// this cast it emits global accesses.  Only for pointers into device allocations (or null).
+template <class T> __device__ __forceinline__ T* gmem(T* p) {
+  return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+
 constexpr int kXcds = 8;
 __device__ __forceinline__ int xcd_block(int b, int nblocks) {
   const int per = nblocks / kXcds, rem = nblocks % kXcds;

@@ -985,7 +985,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_r2c(const BatchDesc* __restr
   cpx<float>* lds = reinterpret_cast<cpx<float>*>(smem);
 #define PM_CASE(NN)                                                                              \
   case NN:                                                                                       \
-    r2c_body<float, float, NN, x_rb<NN>()>(m.rho, m.H, m.Q, m.tw, blk, lds,                      \
+    r2c_body<float, float, NN, x_rb<NN>()>(gmem(m.rho), gmem(m.H), gmem(m.Q), gmem(m.tw), blk, lds, \
                                            lds + (NN / 2) * (x_rb<NN>() + 1));                   \
     break;
   PM_BATCH_CLASS_SWITCH(CLS, m.n, PM_CASE)
@@ -1003,8 +1003,8 @@ __global__ __launch_bounds__(kThreads) void k_batch_c2r(const BatchDesc* __restr
 #define PM_CASE(NN)                                                                              \
   case NN: {                                                                                     \
     cpx<float>* tw = lds + (NN / 2) * (x_rb<NN>() + 1);                                          \
-    c2r_body<float, float, NN, x_rb<NN>()>(m.H, m.Q, reinterpret_cast<float*>(m.H), m.tw, blk,   \
-                                           lds, tw, tw + NN);                                    \
+    c2r_body<float, float, NN, x_rb<NN>()>(gmem(m.H), gmem(m.Q), gmem(reinterpret_cast<float*>(m.H)), \
+                                           gmem(m.tw), blk, lds, tw, tw + NN);                   \
     break;                                                                                       \
   }
   PM_BATCH_CLASS_SWITCH(CLS, m.n, PM_CASE)
@@ -1023,10 +1023,10 @@ __global__ __launch_bounds__(kThreads) void k_batch_pencil(const BatchDesc* __re
   const long blk = b - D->off[ST][k];
   GreenArgs g{};
   if (MODE == 0) {
-    g.coef = D->table[3 * ((long)(*D->step) * D->n_members + k)];
+    g.coef = gmem(D->table)[3 * ((long)(*gmem(D->step)) * D->n_members + k)];
     g.deconvolve = m.deconvolve;
     g.from_grid = m.from_grid;
-    g.fgrid = m.fgrid;
+    g.fgrid = gmem(m.fgrid);
   }
   cpx<float>* lds = reinterpret_cast<cpx<float>*>(smem);
 #define PM_CASE(NN)                                                                              \
@@ -1034,13 +1034,15 @@ __global__ __launch_bounds__(kThreads) void k_batch_pencil(const BatchDesc* __re
     if constexpr (reg_transform<float, NN>()) {                                                  \
       cpx<float>* tw = lds + xslice_elems<float, NN>();                                          \
       float* s2 = reinterpret_cast<float*>(tw + NN);                                             \
-      pencil_reg_body<float, NN, MODE>(m.H, m.Q, MODE == 0 ? m.za : m.ya, MODE == 0 ? m.zb : m.yb, \
-                                       g, m.tw, m.sin2, m.wdec, blk, lds, tw, s2, s2 + NN);      \
+      pencil_reg_body<float, NN, MODE>(gmem(m.H), gmem(m.Q), MODE == 0 ? m.za : m.ya,            \
+                                       MODE == 0 ? m.zb : m.yb, g, gmem(m.tw), gmem(m.sin2),      \
+                                       gmem(m.wdec), blk, lds, tw, s2, s2 + NN);                 \
     } else {                                                                                     \
       cpx<float>* tw = lds + NN * (pencil_w<float, NN>() + 1);                                   \
       float* s2 = reinterpret_cast<float*>(tw + NN);                                             \
-      pencil_body<float, NN, MODE>(m.H, m.Q, MODE == 0 ? m.za : m.ya, MODE == 0 ? m.zb : m.yb, g, \
-                                   m.tw, m.sin2, m.wdec, blk, lds, tw, s2, s2 + NN);             \
+      pencil_body<float, NN, MODE>(gmem(m.H), gmem(m.Q), MODE == 0 ? m.za : m.ya,                \
+                                   MODE == 0 ? m.zb : m.yb, g, gmem(m.tw), gmem(m.sin2),          \
+                                   gmem(m.wdec), blk, lds, tw, s2, s2 + NN);                     \
     }                                                                                            \
     break;                                                                                       \
   }

@@ -1219,26 +1219,28 @@ __global__ __launch_bounds__(kT, TS ? 5 : 1) void k_batch_gkd(const BatchDesc* _
   const int k = batch_member(D, stage, b);
   const BatchMember& m = D->m[k];
   const int tile = (int)(b - D->off[stage][k]);
-  const int step = *D->step;
-  const double* sc = D->table + 3 * ((long)step * D->n_members + k);
+  const int step = *gmem(D->step);
+  const double* sc = gmem(D->table) + 3 * ((long)step * D->n_members + k);
   const float kick = float(sc[1]), drift = float(sc[2]);
   const int in = (m.par + step) & 1;
-  const GkdState<float> ps{m.s[in], m.v[in], m.id[in], m.s[in ^ 1], m.v[in ^ 1], m.id[in ^ 1]};
+  const GkdState<float> ps{gmem(m.s[in]),     gmem(m.v[in]),     gmem(m.id[in]),
+                           gmem(m.s[in ^ 1]), gmem(m.v[in ^ 1]), gmem(m.id[in ^ 1])};
+  const float* phi = gmem(reinterpret_cast<const float*>(m.H));
   if (TS)
-    gkd_body<float, kBT, kBT, kBT, true, true>(ps, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
-                                               m.offsets, m.entries, m.counts, m.capacity, tile, box, hist);
+    gkd_body<float, kBT, kBT, kBT, true, true>(ps, phi, kick, drift, m.L, m.G, gmem(m.offsets), gmem(m.entries),
+                                               gmem(m.counts), m.capacity, tile, box, hist);
   else if (m.local_hist)
-    gkd_body<float, kBT, kBT, kBT, true>(ps, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
-                                         m.offsets, m.entries, m.counts, m.capacity, tile, box, hist);
+    gkd_body<float, kBT, kBT, kBT, true>(ps, phi, kick, drift, m.L, m.G, gmem(m.offsets), gmem(m.entries),
+                                         gmem(m.counts), m.capacity, tile, box, hist);
   else
-    gkd_body<float, kBT, kBT, kBT, false>(ps, reinterpret_cast<const float*>(m.H), kick, drift, m.L, m.G,
-                                          m.offsets, m.entries, m.counts, m.capacity, tile, box, hist);
+    gkd_body<float, kBT, kBT, kBT, false>(ps, phi, kick, drift, m.L, m.G, gmem(m.offsets), gmem(m.entries),
+                                          gmem(m.counts), m.capacity, tile, box, hist);
 }
 
 // state buffer the binning and the deposit read: the gather's output (the scan has advanced
 // the step counter since the gather read it)
 __device__ __forceinline__ int batch_state(const BatchDesc* D, const BatchMember& m) {
-  return (m.par + *D->step) & 1;
+  return (m.par + *gmem(D->step)) & 1;
 }
 
 __global__ __launch_bounds__(kT) void k_batch_deposit(const BatchDesc* __restrict__ D) {
@@ -1249,11 +1251,13 @@ __global__ __launch_bounds__(kT) void k_batch_deposit(const BatchDesc* __restric
   const int cur = batch_state(D, m);
   const int tile = (int)(b - D->off[BS_TILES][k]);
   if (m.det_fix > 0.0)
-    deposit_body<float, kBT, kBT, kBT, true>(m.s[cur], m.id[cur], m.rho, m.mass, m.L, m.G, m.offsets, m.entries,
-                                             m.capacity, tile, acc, m.health, m.det_fix);
+    deposit_body<float, kBT, kBT, kBT, true>(gmem(m.s[cur]), gmem(m.id[cur]), gmem(m.rho), m.mass, m.L, m.G,
+                                             gmem(m.offsets), gmem(m.entries), m.capacity, tile, acc, gmem(m.health),
+                                             m.det_fix);
   else
-    deposit_body<float, kBT, kBT, kBT, false>(m.s[cur], m.id[cur], m.rho, m.mass, m.L, m.G, m.offsets, m.entries,
-                                              m.capacity, tile, acc, m.health);
+    deposit_body<float, kBT, kBT, kBT, false>(gmem(m.s[cur]), gmem(m.id[cur]), gmem(m.rho), m.mass, m.L, m.G,
+                                              gmem(m.offsets), gmem(m.entries), m.capacity, tile, acc,
+                                              gmem(m.health));
 }
 
 // particle-parallel scatter of the members without the tile scatter (BS_SCATTER has no blocks
@@ -1264,8 +1268,9 @@ __global__ __launch_bounds__(kT) void k_batch_scatter(const BatchDesc* __restric
   const int k = batch_member(D, BS_SCATTER, b);
   const BatchMember& m = D->m[k];
   const int cur = batch_state(D, m);
-  bin_scatter_body<float, kBT, kBT, kBT>(m.s[cur], m.id[cur], m.L, m.G, m.offsets, m.cursors, m.entries, m.capacity,
-                                         b - D->off[BS_SCATTER][k], m.health);
+  bin_scatter_body<float, kBT, kBT, kBT>(gmem(m.s[cur]), gmem(m.id[cur]), m.L, m.G, gmem(m.offsets),
+                                         gmem(m.cursors), gmem(m.entries), m.capacity, b - D->off[BS_SCATTER][k],
+                                         gmem(m.health));
 }
 
 __global__ __launch_bounds__(kScatterT) void k_batch_tile_scatter(const BatchDesc* __restrict__ D) {
@@ -1274,9 +1279,9 @@ __global__ __launch_bounds__(kScatterT) void k_batch_tile_scatter(const BatchDes
   const int k = batch_member(D, BS_TILE_SCATTER, b);
   const BatchMember& m = D->m[k];
   const int cur = batch_state(D, m);
-  tile_scatter_body<float, kBT, kBT, kBT>(m.s[cur], m.id[cur], m.L, m.G, m.offsets, m.cursors, m.entries,
-                                          m.capacity, m.health, m.counts, (int)(b - D->off[BS_TILE_SCATTER][k]),
-                                          cur_bins, lim_bins);
+  tile_scatter_body<float, kBT, kBT, kBT>(gmem(m.s[cur]), gmem(m.id[cur]), m.L, m.G, gmem(m.offsets),
+                                          gmem(m.cursors), gmem(m.entries), m.capacity, gmem(m.health),
+                                          gmem(m.counts), (int)(b - D->off[BS_TILE_SCATTER][k]), cur_bins, lim_bins);
 }
 
 __global__ __launch_bounds__(kT) void k_batch_scan_reduce(const BatchDesc* __restrict__ D) {
@@ -1285,8 +1290,9 @@ __global__ __launch_bounds__(kT) void k_batch_scan_reduce(const BatchDesc* __res
   const long b = blockIdx.x;
   const int k = batch_member(D, BS_SCAN, b);
   const BatchMember& m = D->m[k];
-  scan_reduce_spine_body(m.counts, 2L * m.G.ntiles, m.block_sums, m.scan_nb, m.scan_done, b - D->off[BS_SCAN][k],
-                         sh, &last, m.tile_scatter ? m.counts + ts_far_at(m.G.ntiles) : nullptr);
+  uint32_t* counts = gmem(m.counts);
+  scan_reduce_spine_body(counts, 2L * m.G.ntiles, gmem(m.block_sums), m.scan_nb, gmem(m.scan_done),
+                         b - D->off[BS_SCAN][k], sh, &last, m.tile_scatter ? counts + ts_far_at(m.G.ntiles) : nullptr);
 }
 
 __global__ __launch_bounds__(kT) void k_batch_scan_apply(const BatchDesc* __restrict__ D) {
@@ -1294,12 +1300,13 @@ __global__ __launch_bounds__(kT) void k_batch_scan_apply(const BatchDesc* __rest
   const long b = blockIdx.x;
   const int k = batch_member(D, BS_SCAN, b);
   const BatchMember& m = D->m[k];
-  scan_apply_body(m.counts, 2L * m.G.ntiles, m.block_sums, m.offsets, m.cursors, b - D->off[BS_SCAN][k], sh,
-                  m.tile_scatter ? m.counts + ts_far_at(m.G.ntiles) : nullptr,
-                  m.tile_scatter ? m.counts + ts_old_at(m.G.ntiles) : nullptr);
+  uint32_t* counts = gmem(m.counts);
+  scan_apply_body(counts, 2L * m.G.ntiles, gmem(m.block_sums), gmem(m.offsets), gmem(m.cursors),
+                  b - D->off[BS_SCAN][k], sh, m.tile_scatter ? counts + ts_far_at(m.G.ntiles) : nullptr,
+                  m.tile_scatter ? counts + ts_old_at(m.G.ntiles) : nullptr);
   // the last kernel that reads the step's scalars was the gather: advance the counter here, so
   // the binning and the deposit find the gather's output buffer from it
-  if (b == 0 && threadIdx.x == 0) *D->step += 1;
+  if (b == 0 && threadIdx.x == 0) *gmem(D->step) += 1;
 }
 
 }  // namespace
