@@ -131,6 +131,37 @@ def test_ensemble_graph_replays_equal_eager_steps():
     e2.close()
 
 
+def test_deterministic_ensemble_members_equal_solo_runs_bitwise():
+    # With the order-independent deposit nothing in a step depends on the order of work: a
+    # member advanced in the batch (16^3 tiles; the tile scatter for members with 4 or more
+    # tiles per axis, the particle-parallel scatter for the others) then gives the bits of its
+    # solo run in its own tile shape, over a captured graph's replays and launch-by-launch steps.
+    from pmgpu.ensemble import PMEnsemble
+    pairs = []
+    for i in (0, 1, 2, 3):                  # 32^3 / 48^3: particle-parallel, 64^3 / 96^3: tile scatter
+        n, npart, a0, da, flags = MEMBERS[i]
+        pos, vel = make_ics(n, npart, seed=90 + i, box=float(n))
+        pr = (_sim(n, npart, deterministic=True, **flags), _sim(n, npart, deterministic=True, **flags))
+        for s in pr:
+            s.set_state(pos, vel)
+        pairs.append(pr)
+    ens = PMEnsemble([m for _, m in pairs])
+    a0 = [MEMBERS[i][2] for i in (0, 1, 2, 3)]
+    da = [MEMBERS[i][3] for i in (0, 1, 2, 3)]
+    for steps in (11, 20):
+        a_out = ens.run(a0, da, steps)
+        for k, (alone, member) in enumerate(pairs):
+            assert alone.run(a0[k], da[k], steps) == a_out[k]
+            for name, x, y in zip(("x", "v"), alone.get_state(np.float64), member.get_state(np.float64)):
+                assert np.array_equal(x, y), f"{steps} steps, member {k}: {name}"
+            assert np.array_equal(alone.density(np.float64), member.density(np.float64)), f"member {k}: rho"
+        a0 = a_out
+    ens.close()
+    for p in pairs:
+        for s in p:
+            s.close()
+
+
 def test_ensemble_rejects_unsupported_members():
     from pmgpu.ensemble import PMEnsemble
     from pmgpu.sim import PMSim
