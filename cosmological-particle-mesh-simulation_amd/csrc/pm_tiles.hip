@@ -374,19 +374,24 @@ __device__ __forceinline__ void tile_scatter_body(const R* __restrict__ s, const
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const uint32_t e = base + u * NT + threadIdx.x;
-      if (e >= end) continue;
-      if (!(code[u] & kCodeFar)) {
-        const int h0 = 2 * (int)(code[u] & 63u);
-        const int up[3] = {(int)(code[u] >> 6) & 1, (int)(code[u] >> 7) & 1, (int)(code[u] >> 8) & 1};
+      const bool valid = e < end;
+      const bool near = valid && !(code[u] & kCodeFar);
+      const int h0 = 2 * (int)(code[u] & 63u);
+      const int up[3] = {(int)(code[u] >> 6) & 1, (int)(code[u] >> 7) & 1, (int)(code[u] >> 8) & 1};
+      // one LDS atomic per distinct bin of the wavefront (lanes of a tile mostly share their
+      // home bin: per-lane atomics on one address serialize)
 #pragma unroll
-        for (int o = 0; o < 8; ++o) {
-          if (!target_active(up, o)) continue;
-          const int b = corner_bin(h0, o);
-          const uint32_t pos = atomicAdd(&cur[b], 1u);
+      for (int o = 0; o < 8; ++o) {
+        const bool act = near && target_active(up, o);
+        if (!__ballot(act)) continue;  // wave-uniform
+        const int b = corner_bin(h0, o);
+        const uint32_t pos = wave_add<true>(cur, (uint32_t)b, act);
+        if (act) {
           if (pos < lim[b] && pos < capacity) entries[pos] = e;
           else ++lost;
         }
-      } else {
+      }
+      if (valid && (code[u] & kCodeFar)) {
         // moved beyond the neighbourhood (rare): its keys from the stored state
         int c[3], tn[3], up[3];
         R d[3];
