@@ -575,6 +575,34 @@ def test_particles_beyond_the_tile_neighbourhood_are_placed():
         assert np.array_equal(a_, b_), name
 
 
+def test_clustered_particles_step_alike_in_both_scatters():
+    # 32^3 particles packed into a 12^3-cell cube that straddles tile corners: thousands of
+    # particles per tile, so the gather, the deposit and the tile scatter each loop over many
+    # batches of a tile.  The first density matches the oracle's; with the order-independent
+    # deposit three steps in 32 x 8 x 8 tiles (tile scatter) give the bits of three steps in
+    # 8^3 tiles (particle-parallel scatter).
+    from pmgpu.sim import PMSim
+    n, npart = 128, 32
+    rng = np.random.default_rng(91)
+    pos = rng.uniform(26.0, 38.0, size=(3, npart ** 3)).astype(np.float32)
+    vel = rng.normal(0.0, 2.0, size=(3, npart ** 3)).astype(np.float32)
+    kw = dict(omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+    out = []
+    for tiles in ((32, 8, 8), (8, 8, 8)):
+        s = PMSim(n, npart, 32, deterministic=True, **kw)
+        s.set_tiles(*tiles)
+        s.set_state(pos, vel)
+        x0, _ = s.get_state(np.float32)
+        p = O.OracleParams(n_cells=n, n_parts=npart, omega_m0=0.3, omega_lambda0=0.7, h0=0.7)
+        rho_ref = O.density(O.wrap_positions(x0.astype(np.float64), n), p.mass, n)
+        assert rel(s.density(np.float64), rho_ref) < 1e-6, tiles
+        s.run(0.05, 0.01, 3)
+        out.append((*s.get_state(np.float64), s.density(np.float64)))
+        s.close()
+    for name, a_, b_ in zip(("x", "v", "rho"), out[0], out[1]):
+        assert np.array_equal(a_, b_), name
+
+
 # ------------------------------------------------------------------ corrupt state detection
 def test_nonfinite_state_is_reported():
     from pmgpu import _lib
