@@ -16,8 +16,11 @@
 // the rate of ds_add_f32 for this access pattern (measured 0.33 ms against 0.76 ms per
 // 256^3-particle deposit), and the sums are then rounded once instead of at every add.
 // The gather reads the own entries with the tile's potential (plus a -1/+2 stencil margin)
-// staged in LDS.  Counts come from a wave-aggregated atomic per distinct key (one atomic per
-// distinct tile in a wavefront), and the offsets from a three-kernel exclusive scan.
+// staged in LDS, and counts the new positions through an LDS histogram of each tile's 4^3 tile
+// neighbourhood (binning from positions alone counts with a wave-aggregated atomic per distinct
+// key).  A two-kernel exclusive scan turns the counts into offsets; after a gather the tile
+// scatter writes the entries through LDS cursors from ranges the gather reserved, otherwise the
+// particle-parallel scatter through device cursors.
 #include <algorithm>
 #include <cmath>
 #include <vector>
