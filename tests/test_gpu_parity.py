@@ -603,6 +603,33 @@ def test_clustered_particles_step_alike_in_both_scatters():
         assert np.array_equal(a_, b_), name
 
 
+def test_tile_shape_changes_mid_run_keep_the_bits():
+    # The tile scatter keeps its arrays (far counts, reserved range starts, the replaced own
+    # offsets) in the counts buffer after the current shape's counts, so a change of tile shape
+    # moves them.  A deterministic run that switches between shapes with and without the tile
+    # scatter and between tile counts gives the bits of a run in one shape.
+    from pmgpu.sim import PMSim
+    n, npart = 128, 64
+    pos, vel = make_ics(n, npart, seed=97, box=float(2 * n))
+    kw = dict(omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+    ref = PMSim(n, npart, 32, deterministic=True, **kw)
+    ref.set_tiles(32, 8, 8)
+    ref.set_state(pos, vel)
+    a_ref = ref.run(0.05, 0.01, 20)
+    s = PMSim(n, npart, 32, deterministic=True, **kw)
+    s.set_state(pos, vel)
+    a = 0.05
+    for tiles in ((32, 8, 8), (8, 8, 8), (16, 16, 16), (32, 16, 8), (32, 8, 8)):
+        s.set_tiles(*tiles)
+        a = s.run(a, 0.01, 4)
+    assert a == a_ref
+    for name, x, y in zip(("x", "v"), s.get_state(np.float64), ref.get_state(np.float64)):
+        assert np.array_equal(x, y), name
+    assert np.array_equal(s.density(np.float64), ref.density(np.float64))
+    s.close()
+    ref.close()
+
+
 # ------------------------------------------------------------------ corrupt state detection
 def test_nonfinite_state_is_reported():
     from pmgpu import _lib
