@@ -216,6 +216,24 @@ def test_full_size_deterministic_run_is_bitwise_repeatable(ics):
     np.testing.assert_allclose(out[0][2], out[1][2], rtol=1e-13, atol=0)
 
 
+def test_full_size_tile_scatter_gives_the_bits_of_the_particle_parallel_scatter(ics):
+    # the workload's own shape and tile shape: 40 steps in 32 x 8 x 8 tiles (the tile scatter,
+    # its reserved ranges and bin codes) against 40 steps in 8^3 tiles (the particle-parallel
+    # scatter with device cursors), both with the order-independent deposit
+    from pmgpu.sim import PMSim
+    pos, vel = ics
+    out = []
+    for tiles in ((32, 8, 8), (8, 8, 8)):
+        s = PMSim(N, NP, 32, deterministic=True, omega_m0=0.3, omega_lambda0=0.7, omega_k0=0.0, h0=0.7)
+        s.set_tiles(*tiles)
+        s.set_state(pos, vel)
+        s.run(A0, DA, 40)
+        out.append(s.get_state(np.float32))
+        s.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
 def test_full_size_step_matches_cpu_port(ics):
     # density.py:7-48, potential.py:7-29 and integrate.py:9-97 through the port, one step
     import pm_oracle as O
